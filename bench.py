@@ -1,0 +1,165 @@
+"""FDTD throughput benchmark on one MI355X: Mcell-updates/s and % of the HBM roofline.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload layered|boxes|plain]
+
+A step is one FDTD time step (H and E curl updates with CPML, sparse fixes, port/probe
+observers) over the whole grid. The default workload is the single large throughput run:
+768x768x384 nodes (226.5 M), 1 mm, 16-cell CPML, eps_r ~ U[1,4] per 8-cell z-layer, two
+50-ohm z-ports. Inputs are resident in HBM before the timed region starts. Prints one JSON
+line with `roofline` (dominant kernel, HIP-event timed) and `cpu_baseline` (the oracle's
+FP32 restatement of the reference CPU path on a bounded sample of the same recipe).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-updates/s and % HBM roofline at 1 GPU; batched sweep speedup; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(name):
+    from fieldgrid_amd import workloads as W
+    if name == "layered":
+        sim = W.layered()
+        pulse = W.layered_pulse()
+        extras_kw = {}
+        config = {"workload": "layered throughput run", "dims": [768, 768, 384], "dx_mm": 1.0, "pml": 16,
+                  "layers": "eps_r~U[1,4] per 8-cell z-layer, seed 3", "ports": 2}
+    elif name == "boxes":
+        sim = W.boxes()
+        pulse = W.boxes_pulse()
+        extras_kw = {"boxes": True}
+        config = {"workload": "boxes + Debye + port + NTFF", "dims": [256, 256, 256], "dx_mm": 0.5, "pml": 12,
+                  "boxes": 40, "debye_boxes": 10, "ntff_freqs_ghz": [2, 5, 10]}
+    elif name == "plain":
+        sim, probes = W.plain()
+        pulse = W.plain_pulse()
+        extras_kw = {"probes": probes}
+        config = {"workload": "plain vacuum", "dims": [64, 64, 64], "dx_mm": 1.0, "pml": 10, "probes": 16}
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return sim, pulse, extras_kw, config
+
+
+def make_extras(sim, steps, extras_kw):
+    import math
+    from fieldgrid_amd import fdtd as F
+    from fieldgrid_amd import workloads as W
+    if extras_kw.get("boxes"):
+        return W.boxes_extras(sim, steps=steps, check_every=0)
+    return F.Extras(max_steps=steps, decay_db=math.inf, probes=extras_kw.get("probes", []))
+
+
+def cpu_baseline(name, budget_s=20.0):
+    """The oracle's FP32 restatement of run_cpu (cpu.rs:424-482) on the host cores, on a
+    bounded sample of the same recipe. Returns the dict for the JSON line."""
+    import math
+    from fieldgrid_amd import fdtd as F
+    from fieldgrid_amd import workloads as W
+    from oracle import oracle as O
+    if name == "layered":
+        sim = W.layered(dims=(96, 192, 384))
+        pulse = W.layered_pulse()
+        sample = "96x192x384 slab of the layered recipe (same CPML 16, layering, ports)"
+        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf)
+    elif name == "boxes":
+        sim = W.boxes(nodes=96)
+        pulse = W.boxes_pulse()
+        sample = "96^3 version of the boxes recipe (CPML 12, NTFF, Debye)"
+        extras = lambda k: W.boxes_extras(sim, steps=k)
+    else:
+        sim, probes = W.plain()
+        pulse = W.plain_pulse()
+        sample = "the full 64^3 plain workload"
+        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf, probes=probes)
+    n = sim.dims()
+    nn = n[0] * n[1] * n[2]
+    t = time.perf_counter()
+    O.run_cpu(sim.desc, 0, pulse, extras(2), 32, fields=False)  # warm-up and size probe
+    per_step = max((time.perf_counter() - t) / 2, 1e-6)
+    k = max(2, min(2000, int(budget_s / per_step)))
+    t = time.perf_counter()
+    r = O.run_cpu(sim.desc, 0, pulse, extras(k), 32, fields=False)
+    dt = time.perf_counter() - t
+    return {"value": round(nn * r.steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": O.threads(),
+            "kind": "port", "sample": f"{sample}: {r.steps} steps, {nn} nodes, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain"])
+    ap.add_argument("--engine", default="fused", choices=["fused", "split"])
+    ap.add_argument("--timed-steps", type=int, default=50, help="steps timed per kernel with HIP events")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU halo exchange is not "
+              "part of this build", file=sys.stderr)
+        return 2
+    from fieldgrid_amd import abi
+    from fieldgrid_amd import fdtd as F
+    from fieldgrid_amd import workloads as W
+
+    t_build = time.perf_counter()
+    sim, pulse, extras_kw, config = build_workload(args.workload)
+    build_s = time.perf_counter() - t_build
+    n = sim.dims()
+    nn = n[0] * n[1] * n[2]
+    total = args.warmup + args.steps + args.timed_steps
+    extras = make_extras(sim, total, extras_kw)
+    extras.engine = abi.FG_ENGINE_SPLIT if args.engine == "split" else abi.FG_ENGINE_FUSED
+    bpn = W.bytes_per_node_step(sim, extras)
+    gpu = F.Gpu(0)
+    sess = F.Session(gpu, sim, 0, pulse, extras)
+    sess.step(args.warmup)
+    sess.sync()
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    sess.step(args.steps)
+    sess.sync()
+    gpu.synchronize()
+    secs = time.perf_counter() - t0
+    # per-kernel HIP-event timing on the session's stream, after the timed region
+    sess.timing(True)
+    sess.step(args.timed_steps)
+    ms, timed = sess.timing(False)
+    energy = sess.energy()
+    sess.close()
+    if not (energy == energy) or energy == float("inf"):
+        print("the FDTD run went unstable", file=sys.stderr)
+        return 1
+    value = nn * args.steps / secs / 1e6
+    sweep_ms = ms[0] / max(timed, 1)
+    # algorithmic bytes of the dominant (sweep) kernel per launch: E/H/coef/psi traffic only
+    core_bpn = W.bytes_per_node_step(sim, None, core=True)
+    achieved = core_bpn * nn / (sweep_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(secs * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded materials and ports)",
+        "config": dict(config, engine=args.engine, nodes=nn, bytes_per_node_step=round(bpn, 3),
+                       hbm_frac_whole_step=round(bpn * nn * args.steps / secs / 1e9 / HBM_PEAK_GBS, 4),
+                       build_s=round(build_s, 2), gpu=gpu.name),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_fused" if args.engine == "fused" else "k_update_h+k_update_e",
+                     "kernel_ms": round(sweep_ms, 4), "alg_bytes_per_node": round(core_bpn, 3),
+                     "fix_ms": round(ms[1] / max(timed, 1), 4), "observe_ms": round(ms[2] / max(timed, 1), 4)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.workload)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,48 @@
+"""Build recipe for libfieldgrid.so (gfx950 HIP engine + host problem builder).
+
+Run `python -m fieldgrid_amd.build` (or __graft_entry__.build()). The library is built in
+place, next to this file, so it travels with the source tree.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfieldgrid.so")
+ARCH = os.environ.get("FIELDGRID_ARCH", "gfx950")
+
+# Packed-FP32 instructions are switched off for all device code (-packed-fp32-ops).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall",
+              "-Wno-missing-field-initializers"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    """Compile csrc/ into libfieldgrid.so (always recompiles)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    tmp = os.path.join(HERE, "_objs")
+    os.makedirs(tmp, exist_ok=True)
+    eng = os.path.join(tmp, "engine.o")
+    sim = os.path.join(tmp, "sim.o")
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng])
+    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sim.cpp"), "-o", sim])
+    out_tmp = LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, sim, "-o", out_tmp, "-lpthread"])
+    os.replace(out_tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,139 @@
+"""GPU parity: the HIP engine through the C-ABI against the oracle on the same inputs.
+
+Tolerances: the reference's own CPU-vs-GPU bar, worst = max|a-b|/max|a| < 1e-5 on port
+series, plane DFT and NTFF (crates/sims-field/tests/fdtd.rs:127-174), and relative L2 < 1e-5
+on the final E and H fields against the FP32 oracle. Errors against the FP64 oracle are
+printed and bounded at 1e-4 (the FP32 path itself differs from FP64 by ~1e-6).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fieldgrid_amd import abi
+from fieldgrid_amd import fdtd as F
+from fieldgrid_amd import workloads as W
+from oracle import oracle as O
+
+from .fixtures import everything, everything_extras, rel_l2, worst
+
+ENGINES = [abi.FG_ENGINE_FUSED, abi.FG_ENGINE_SPLIT]
+NAMES = {abi.FG_ENGINE_FUSED: "fused", abi.FG_ENGINE_SPLIT: "split"}
+
+
+def _compare(rec, o32, o64, what, limit=1e-5):
+    print(f"\n{what}: steps {rec.steps}")
+    for name in ("series", "plane", "ntff", "probes"):
+        a, b = getattr(o32, name), getattr(rec, name)
+        if len(a) == 0:
+            continue
+        w = worst(a, b)
+        w64 = worst(getattr(o64, name), b) if o64 is not None else float("nan")
+        print(f"  {name}: worst vs fp32 oracle {w:.2e}, vs fp64 {w64:.2e}")
+        assert w < limit, f"{what} {name}: {w}"
+        if o64 is not None:
+            assert w64 < 1e-4
+    if rec.e is not None and o32.e is not None:
+        for name in ("e", "h"):
+            l2 = rel_l2(getattr(o32, name), getattr(rec, name))
+            l64 = rel_l2(getattr(o64, name), getattr(rec, name)) if o64 is not None else float("nan")
+            print(f"  {name}: rel L2 vs fp32 oracle {l2:.2e}, vs fp64 {l64:.2e}")
+            assert l2 < 1e-5, f"{what} {name}: {l2}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
+def test_plain_vacuum_matches_the_oracle(gpu, engine):
+    """The 64^3 plain workload in full: 500 steps, Ez/Hy at 8 probe nodes, final fields."""
+    sim, probes = W.plain()
+    ex = F.Extras(max_steps=500, decay_db=math.inf, check_every=100, probes=probes, engine=engine)
+    pulse = W.plain_pulse()
+    rec = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    o32 = O.run_cpu(sim.desc, 0, pulse, ex, 32)
+    o64 = O.run_cpu(sim.desc, 0, pulse, ex, 64)
+    assert rec.steps == 500
+    assert np.abs(rec.probes).max() > 0
+    _compare(rec, o32, o64, f"plain {NAMES[engine]}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("driven", [0, 1])
+@pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
+def test_every_device_feature_matches_the_oracle(gpu, engine, driven):
+    """tests/fdtd.rs:147-174 without sheets and wires (which the engine refuses): Debye band
+    poles, PEC, C/R/L lumped, two multi-column ports, plane DFT, NTFF; both driven ports."""
+    sim = everything(sheets=False, wires=False)
+    ex = everything_extras(sim, 1500)
+    ex.engine = engine
+    pulse = F.Pulse(5e9, 5e9)
+    rec = F.run(gpu, sim, driven, pulse, ex, fields=True)
+    o32 = O.run_cpu(sim.desc, driven, pulse, ex, 32)
+    o64 = O.run_cpu(sim.desc, driven, pulse, ex, 64)
+    assert rec.steps == 1500 and o32.steps == 1500
+    _compare(rec, o32, o64, f"everything {NAMES[engine]} driven {driven}")
+
+
+@pytest.mark.gpu
+def test_fused_and_split_engines_agree(gpu):
+    """tests/fdtd.rs:164-167: separate against fused, within 1e-5."""
+    sim = everything(sheets=False, wires=False)
+    pulse = F.Pulse(5e9, 5e9)
+    recs = []
+    for engine in ENGINES:
+        ex = everything_extras(sim, 1500)
+        ex.engine = engine
+        recs.append(F.run(gpu, sim, 0, pulse, ex))
+    for name in ("series", "plane", "ntff"):
+        assert worst(getattr(recs[1], name), getattr(recs[0], name)) < 1e-5
+
+
+@pytest.mark.gpu
+def test_sheets_and_wires_are_refused_so_the_caller_can_fall_back(gpu):
+    sim = everything()
+    ex = everything_extras(sim, 10)
+    with pytest.raises(abi.FieldgridError) as e:
+        F.run(gpu, sim, 0, F.Pulse(5e9, 5e9), ex)
+    assert e.value.status == abi.FG_UNSUPPORTED
+
+
+@pytest.mark.gpu
+def test_a_run_stops_when_progress_says_so(gpu):
+    """tests/fdtd.rs:176-187 on the device path."""
+    sim = everything(sheets=False, wires=False)
+    calls = []
+
+    def progress(steps, decay):
+        calls.append(steps)
+        return len(calls) < 2
+
+    with pytest.raises(abi.FieldgridError) as e:
+        F.run(gpu, sim, 0, F.Pulse(5e9, 5e9), everything_extras(sim, 5000), progress)
+    assert str(e.value) == "cancelled" and e.value.status == abi.FG_CANCELLED
+    assert calls == [250, 500]
+
+
+@pytest.mark.gpu
+def test_a_run_past_the_courant_limit_is_reported_unstable(gpu):
+    """run.rs:736-738: raising dt above the Courant limit makes the energy blow up."""
+    sim, _ = W.plain(nodes=24, pml=4)
+    p = sim.problem()
+    p.dt = sim.dt * 1.5  # the reference's `pub dt` (sim.rs:176), edited after packing
+    ex = F.Extras(max_steps=2000, decay_db=math.inf, check_every=200)
+    keep = []
+    x = ex.to_c(keep)
+    rec, bufs = F._record_buffers(sim, ex, False)
+    pl = abi.fg_pulse(10e9, 5e9)
+    st = abi.lib().fg_run(gpu._h, p, 0, pl, x, abi.PROGRESS_FN(lambda u, s, d: 1), None, rec)
+    assert st == abi.FG_UNSTABLE
+    assert abi.lib().fg_last_error().decode() == "the FDTD run went unstable"
+
+
+@pytest.mark.gpu
+def test_decay_stops_a_run_early_like_the_oracle(gpu):
+    """run.rs:745-747: a finite decay_db stops after pulse_end once the energy has decayed."""
+    sim, _ = W.plain(nodes=32, pml=8)
+    ex = F.Extras(max_steps=20000, decay_db=30.0, check_every=100)
+    rec = F.run(gpu, sim, 0, W.plain_pulse(), ex)
+    o = O.run_cpu(sim.desc, 0, W.plain_pulse(), ex, 32, fields=False)
+    assert rec.steps == o.steps < 20000
+    assert abs(rec.decay_db - o.decay_db) < 0.5
