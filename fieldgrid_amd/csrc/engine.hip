@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -192,65 +193,216 @@ __global__ __launch_bounds__(256) void k_update_e(Geo g, float *__restrict__ E, 
     E[2 * nn + id] = ez;
 }
 
-// ---- fused engine: one H->E pass per step, ping-pong buffers (fdtd.wgsl:186-330) ----------
+// ---- fused engine: one H->E pass per step, ping-pong buffers ------------------------------
 //
-// A block is a 64 (k) x TJ (j) tile that marches over `seg` planes of i. For each plane it
-// computes new H for its nodes (written to Hn and to LDS), recomputes new H on the j-1 row
-// and k-1 column halo from the old buffers, and then new E from LDS; the i-1 plane's new
-// Hy/Hz ride in registers. Halo recomputes read only old buffers and write nothing, so no
-// block ever reads what another block writes in the same step.
-template <int TJ>
-__global__ __launch_bounds__(64 * TJ) void k_fused(Geo g, const float *__restrict__ Eo, float *__restrict__ En,
-                                                   const float *__restrict__ Ho, float *__restrict__ Hn,
-                                                   const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg) {
-    __shared__ float hs[2][3][TJ + 1][65];
-    const uint32_t tk = threadIdx.x, tj = threadIdx.y;
-    const uint32_t k0 = blockIdx.x * 64, j0 = blockIdx.y * TJ;
-    const uint32_t k = k0 + tk, j = j0 + tj;
-    const uint32_t i0 = blockIdx.z * seg, i1 = min(i0 + seg, g.n0);
-    const bool live = k < g.n2 && j < g.n1;
-    const uint64_t nn = g.nn;
-    float prev_hy = 0.f, prev_hz = 0.f;
-    if (live && i0 > 0) {
-        prev_hy = h_next<1>(g, Eo, Ho, psio, nullptr, i0 - 1, j, k);
-        prev_hz = h_next<2>(g, Eo, Ho, psio, nullptr, i0 - 1, j, k);
+// The reference fuses H and E per step with 64x4 tiles and recomputed halos
+// (fdtd.wgsl:186-330). Here a block is a column of ROWS = TJ+2 waves marching along i (the
+// slowest axis) over `seg` planes:
+//   * wave w holds row j = j0-1+w; rows 1..TJ are owned, row 0 recomputes new Hx/Hz at
+//     j0-1 (the E update needs them), row TJ+1 only supplies E at j0+TJ (the H update needs it);
+//   * lane L holds k = k0-1+L with k0 = 62*blockIdx.x; lanes 1..62 are owned, lanes 0 and 63
+//     are k-halos whose values reach their neighbours through cross-lane shuffles;
+//   * E(i) and new Hy/Hz(i-1) ride in registers from one plane to the next, so each plane
+//     loads E(i+1) and H(i) once, and rows trade E and new H through LDS behind one barrier.
+// Halo work reads only the old buffers and writes nothing, so no block reads what another
+// block writes in the same step. Every node's own E, H, coefficient index and CPML psi are
+// read once and written once per step.
+constexpr int OWN_K = 62;
+
+// Block coordinates of one march block (k tile, j tile, i segment). Blocks that touch a CPML
+// slab and blocks that do not are launched as two lists, so the interior kernel carries no
+// psi code at all.
+struct MarchBlock {
+    uint32_t bx, by, bz;
+};
+
+// One CPML psi recursion, psi = b*psi + c*diff*raw (cpu.rs:78-99, 137-157), old -> new store.
+__device__ __forceinline__ float cpml(const float *__restrict__ po, float *__restrict__ pn, uint64_t q, float b,
+                                      float c, float raw, float diff, bool store) {
+    const float r = b * po[q] + c * diff * raw;
+    if (store) pn[q] = r;
+    return r;
+}
+
+template <int TJ, bool PML>
+__global__ __launch_bounds__(64 * (TJ + 2)) void k_march(Geo g, const MarchBlock *__restrict__ blocks,
+                                                         const float *__restrict__ Eo, float *__restrict__ En,
+                                                         const float *__restrict__ Ho, float *__restrict__ Hn,
+                                                         const float *__restrict__ psio, float *__restrict__ psin,
+                                                         uint32_t seg) {
+    constexpr int ROWS = TJ + 2;
+    __shared__ float xe[2][ROWS][2][64];  // E(i+1) Ex, Ez of each row, for the row above
+    __shared__ float xh[2][ROWS][2][64];  // new H(i) Hx, Hz of each row, for the row below
+    const MarchBlock mb = blocks[blockIdx.x];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int k = int(mb.bx) * OWN_K + lane - 1;
+    const int j = int(mb.by) * TJ + w - 1;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 1 && lane <= OWN_K && w >= 1 && w <= TJ;
+    const bool bottom = w == TJ + 1;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;  // k+1 / j+1 inside the grid
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    // CPML constants of this lane (k) and row (j); psi offsets that do not depend on i
+    int shz = -1, sez = -1, shy = -1, sey = -1;
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = 0, oze = 0, oyh = 0, oye = 0;
+    const uint32_t p2 = 2u * g.pml;
+    if constexpr (PML) {
+        shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        ozh = uj * p2 + uint32_t(max(shz, 0)), oze = uj * p2 + uint32_t(max(sez, 0));
+        oyh = uint32_t(max(shy, 0)) * g.n2 + uk, oye = uint32_t(max(sey, 0)) * g.n2 + uk;
     }
-    for (uint32_t i = i0; i < i1; i++) {
+    const uint16_t *idx16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    const uint32_t istart = i0 > 0 ? i0 - 1 : 0;
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+    if (act) {
+        const float *p = Eo + istart * plane + off;
+        ex = p[0], ey = p[nn], ez = p[2 * nn];
+    }
+    // E(istart) Ex/Ez of the row below, through the buffer the first plane does not write
+    {
+        const int pb = (istart + 1) & 1;
+        xe[pb][w][0][lane] = ex;
+        xe[pb][w][1][lane] = ez;
+    }
+    __syncthreads();
+    float exn = 0.f, ezn = 0.f;
+    if (w + 1 < ROWS) {
+        const int pb = (istart + 1) & 1;
+        exn = xe[pb][w + 1][0][lane];
+        ezn = xe[pb][w + 1][1][lane];
+    }
+    float hy_p = 0.f, hz_p = 0.f;
+    for (uint32_t i = istart; i < i1; i++) {
         const int b = i & 1;
-        const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+        const bool write = own && i >= i0;
+        const bool il = i + 1 < g.n0;
+        const uint64_t base = i * plane + off;
+        float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f;
+        if (act && il) {
+            const float *p = Eo + base + plane;
+            ex1 = p[0], ey1 = p[nn], ez1 = p[2 * nn];
+        }
+        const float idx_ = g.ax[0].inv_d[i], iddx = g.ax[0].inv_dd[i];
+        // CPML values of plane i and the psi bases of this plane
+        int shx = -1, sex = -1;
+        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
+        uint64_t zrow = 0, yrow = 0, xho = 0, xeo = 0;
+        if constexpr (PML) {
+            shx = g.ax[0].slot_h[i], sex = g.ax[0].slot_e[i];
+            bhx = g.ax[0].bh[i], chx = g.ax[0].ch[i], rdx = g.ax[0].raw_d[i];
+            bex = g.ax[0].be[i], cex = g.ax[0].ce[i], rddx = g.ax[0].raw_dd[i];
+            zrow = uint64_t(i) * g.n1 * p2;             // a = 2: (i*n1 + j)*2p + s
+            yrow = uint64_t(i) * p2 * g.n2;             // a = 1: (i*2p + s)*n2 + k
+            xho = uint64_t(max(shx, 0)) * plane + off;  // a = 0: (s*n1 + j)*n2 + k
+            xeo = uint64_t(max(sex, 0)) * plane + off;
+        }
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+        // k+1 neighbours of E(i) from the next lane
+        const float ex_k = __shfl_down(ex, 1, 64), ey_k = __shfl_down(ey, 1, 64);
         float hx = 0.f, hy = 0.f, hz = 0.f;
-        if (live) {
-            hx = h_next<0>(g, Eo, Ho, psio, psin, i, j, k);
-            hy = h_next<1>(g, Eo, Ho, psio, psin, i, j, k);
-            hz = h_next<2>(g, Eo, Ho, psio, psin, i, j, k);
-            Hn[id] = hx;
-            Hn[nn + id] = hy;
-            Hn[2 * nn + id] = hz;
-            hs[b][0][tj + 1][tk + 1] = hx;
-            hs[b][1][tj + 1][tk + 1] = hy;
-            hs[b][2][tj + 1][tk + 1] = hz;
-            if (tj == 0 && j > 0) {
-                hs[b][0][0][tk + 1] = h_next<0>(g, Eo, Ho, psio, nullptr, i, j - 1, k);
-                hs[b][2][0][tk + 1] = h_next<2>(g, Eo, Ho, psio, nullptr, i, j - 1, k);
+        if (act && !bottom) {
+            const float *ph = Ho + base;
+            const float hxo = ph[0], hyo = ph[nn], hzo = ph[2 * nn];
+            // Hx: u = y, v = z
+            hx = hxo;
+            if (jl && kl) {
+                const float ev = ezn - ez, eu = ey_k - ey;
+                float t1 = ev * idy, t2 = eu * idz;
+                if (PML && shy >= 0) t1 += cpml(psio, psin, PSI(1, 0, 1) + yrow + oyh, bhy, chy, rdy, ev, write);
+                if (PML && shz >= 0) t2 += cpml(psio, psin, PSI(1, 0, 2) + zrow + ozh, bhz, chz, rdz, eu, write);
+                hx = hxo - g.k_mu * (t1 - t2);
+            }
+            // Hy: u = z, v = x
+            hy = hyo;
+            if (kl && il) {
+                const float ev = ex_k - ex, eu = ez1 - ez;
+                float t1 = ev * idz, t2 = eu * idx_;
+                if (PML && shz >= 0) t1 += cpml(psio, psin, PSI(1, 1, 2) + zrow + ozh, bhz, chz, rdz, ev, write);
+                if (PML && shx >= 0) t2 += cpml(psio, psin, PSI(1, 1, 0) + xho, bhx, chx, rdx, eu, write);
+                hy = hyo - g.k_mu * (t1 - t2);
+            }
+            // Hz: u = x, v = y
+            hz = hzo;
+            if (il && jl) {
+                const float ev = ey1 - ey, eu = exn - ex;
+                float t1 = ev * idx_, t2 = eu * idy;
+                if (PML && shx >= 0) t1 += cpml(psio, psin, PSI(1, 2, 0) + xho, bhx, chx, rdx, ev, write);
+                if (PML && shy >= 0) t2 += cpml(psio, psin, PSI(1, 2, 1) + yrow + oyh, bhy, chy, rdy, eu, write);
+                hz = hzo - g.k_mu * (t1 - t2);
+            }
+            if (write) {
+                float *q = Hn + base;
+                q[0] = hx, q[nn] = hy, q[2 * nn] = hz;
             }
         }
-        if (tj == TJ - 1 && tk < TJ) {
-            const uint32_t cj = j0 + tk;
-            if (k0 > 0 && cj < g.n1) {
-                hs[b][0][tk + 1][0] = h_next<0>(g, Eo, Ho, psio, nullptr, i, cj, k0 - 1);
-                hs[b][1][tk + 1][0] = h_next<1>(g, Eo, Ho, psio, nullptr, i, cj, k0 - 1);
-            }
-        }
+        xh[b][w][0][lane] = hx;
+        xh[b][w][1][lane] = hz;
+        xe[b][w][0][lane] = ex1;
+        xe[b][w][1][lane] = ez1;
         __syncthreads();
-        if (live) {
-            const float hz_j = hs[b][2][tj][tk + 1], hx_j = hs[b][0][tj][tk + 1];
-            const float hy_k = hs[b][1][tj + 1][tk], hx_k = hs[b][0][tj + 1][tk];
-            En[id] = e_next<0>(g, Eo[id], hz - hz_j, hy - hy_k, i, j, k, id, psio, psin);
-            En[nn + id] = e_next<1>(g, Eo[nn + id], hx - hx_k, hz - prev_hz, i, j, k, id, psio, psin);
-            En[2 * nn + id] = e_next<2>(g, Eo[2 * nn + id], hy - prev_hy, hx - hx_j, i, j, k, id, psio, psin);
-            prev_hy = hy;
-            prev_hz = hz;
+        const float hx_j = w > 0 ? xh[b][w - 1][0][lane] : 0.f, hz_j = w > 0 ? xh[b][w - 1][1][lane] : 0.f;
+        const float exn1 = w + 1 < ROWS ? xe[b][w + 1][0][lane] : 0.f;
+        const float ezn1 = w + 1 < ROWS ? xe[b][w + 1][1][lane] : 0.f;
+        // k-1 neighbours of new H(i) from the previous lane
+        const float hx_k = __shfl_up(hx, 1, 64), hy_k = __shfl_up(hy, 1, 64);
+        if (write) {
+            float *q = En + base;
+            // Ex: u = y, v = z
+            {
+                const float2 cf = g.coef_sets[g.wide ? g.coef_index[base] : idx16[base]];
+                float e = ex;
+                if (cf.y != 0.f) {
+                    const float hv = hz - hz_j, hu = hy - hy_k;
+                    float t1 = hv * iddy, t2 = hu * iddz;
+                    if (PML && sey >= 0) t1 += cpml(psio, psin, PSI(0, 0, 1) + yrow + oye, bey, cey, rddy, hv, true);
+                    if (PML && sez >= 0) t2 += cpml(psio, psin, PSI(0, 0, 2) + zrow + oze, bez, cez, rddz, hu, true);
+                    e = cf.x * ex + cf.y * (t1 - t2);
+                }
+                q[0] = e;
+            }
+            // Ey: u = z, v = x
+            {
+                const float2 cf = g.coef_sets[g.wide ? g.coef_index[nn + base] : idx16[nn + base]];
+                float e = ey;
+                if (cf.y != 0.f) {
+                    const float hv = hx - hx_k, hu = hz - hz_p;
+                    float t1 = hv * iddz, t2 = hu * iddx;
+                    if (PML && sez >= 0) t1 += cpml(psio, psin, PSI(0, 1, 2) + zrow + oze, bez, cez, rddz, hv, true);
+                    if (PML && sex >= 0) t2 += cpml(psio, psin, PSI(0, 1, 0) + xeo, bex, cex, rddx, hu, true);
+                    e = cf.x * ey + cf.y * (t1 - t2);
+                }
+                q[nn] = e;
+            }
+            // Ez: u = x, v = y
+            {
+                const float2 cf = g.coef_sets[g.wide ? g.coef_index[2 * nn + base] : idx16[2 * nn + base]];
+                float e = ez;
+                if (cf.y != 0.f) {
+                    const float hv = hy - hy_p, hu = hx - hx_j;
+                    float t1 = hv * iddx, t2 = hu * iddy;
+                    if (PML && sex >= 0) t1 += cpml(psio, psin, PSI(0, 2, 0) + xeo, bex, cex, rddx, hv, true);
+                    if (PML && sey >= 0) t2 += cpml(psio, psin, PSI(0, 2, 1) + yrow + oye, bey, cey, rddy, hu, true);
+                    e = cf.x * ez + cf.y * (t1 - t2);
+                }
+                q[2 * nn] = e;
+            }
         }
+#undef PSI
+        ex = ex1, ey = ey1, ez = ez1;
+        exn = exn1, ezn = ezn1;
+        hy_p = hy, hz_p = hz;
     }
 }
 
@@ -531,6 +683,7 @@ struct Session {
     DevBuf<uint64_t> probe_f;
     DevBuf<uint32_t> patch;
     DevBuf<double> partial;
+    DevBuf<MarchBlock> blk_int, blk_pml;  // march blocks clear of / touching the CPML slabs
     std::vector<hipEvent_t> events;
 
     ~Session() {
@@ -581,6 +734,36 @@ static uint64_t psi_len(const uint32_t n[3], int a, uint32_t pml) {
     uint64_t m[3] = {n[0], n[1], n[2]};
     m[a] = 2ull * pml;
     return m[0] * m[1] * m[2];
+}
+
+// Split the march blocks into those whose reach (their own nodes plus the one-node halo on
+// every side) touches a CPML slab position and those that do not.
+static void plan_march(Session &s, const fg_problem *P, const std::vector<int> &slots, uint32_t nmax) {
+    std::vector<uint32_t> pre[3];
+    for (int a = 0; a < 3; a++) {
+        pre[a].assign(P->n[a] + 1, 0);
+        for (uint32_t q = 0; q < P->n[a]; q++) {
+            const bool slab = P->pml && (slots[(size_t(a) * 2) * nmax + q] >= 0 || slots[(size_t(a) * 2 + 1) * nmax + q] >= 0);
+            pre[a][q + 1] = pre[a][q] + (slab ? 1 : 0);
+        }
+    }
+    auto touches = [&](int a, int64_t lo, int64_t hi) {  // any slab position in [lo, hi]
+        lo = std::max<int64_t>(lo, 0);
+        hi = std::min<int64_t>(hi, int64_t(P->n[a]) - 1);
+        return hi >= lo && pre[a][size_t(hi) + 1] > pre[a][size_t(lo)];
+    };
+    const uint32_t gx = (P->n[2] + OWN_K - 1) / OWN_K, gy = (P->n[1] + s.tj - 1) / s.tj, gz = (P->n[0] + s.seg - 1) / s.seg;
+    std::vector<MarchBlock> in, pm;
+    for (uint32_t bz = 0; bz < gz; bz++)
+        for (uint32_t by = 0; by < gy; by++)
+            for (uint32_t bx = 0; bx < gx; bx++) {
+                const int64_t k0 = int64_t(bx) * OWN_K, j0 = int64_t(by) * s.tj, i0 = int64_t(bz) * s.seg;
+                const bool slab = touches(2, k0 - 1, k0 + OWN_K) || touches(1, j0 - 1, j0 + s.tj) ||
+                                  touches(0, i0 - 1, i0 + s.seg);
+                (slab ? pm : in).push_back(MarchBlock{bx, by, bz});
+            }
+    s.blk_int.upload(in.data(), in.size(), s.stream);
+    s.blk_pml.upload(pm.data(), pm.size(), s.stream);
 }
 
 static Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse,
@@ -748,9 +931,13 @@ static Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven
     o.step = s.step.p;
     s.obs_blocks = o.blk_ports + o.blk_probes + o.blk_plane + o.blk_ntff;
     s.partial.alloc(ENERGY_BLOCKS);
-    // fused tile: 64 x TJ, marching `seg` planes of i
-    s.tj = 4;
-    s.seg = 8;
+    // fused march: TJ owned rows (+2 halo rows) x 62 owned lanes, `seg` planes of i per block
+    s.tj = 8;
+    s.seg = 64;
+    if (const char *v = getenv("FIELDGRID_TJ")) s.tj = uint32_t(atoi(v));
+    if (const char *v = getenv("FIELDGRID_SEG")) s.seg = std::max(1u, uint32_t(atoi(v)));
+    if (s.tj != 4 && s.tj != 8 && s.tj != 14) s.tj = 8;
+    if (s.fused) plan_march(s, P, axs, nmax);
     FG_CHECK(hipStreamSynchronize(st));
     return S.release();
 }
@@ -789,14 +976,22 @@ static void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     float *Ecur;
     const float *Eold = nullptr;
     if (s.fused) {
-        dim3 block(64, s.tj);
-        dim3 grid((g.n2 + 63) / 64, (g.n1 + s.tj - 1) / s.tj, (g.n0 + s.seg - 1) / s.seg);
+        dim3 block(64, s.tj + 2);
+        const float *Eo = s.E[a].p, *Ho = s.H[a].p, *Po = s.psi[a].p;
+        float *En = s.E[b].p, *Hn = s.H[b].p, *Pn = s.psi[b].p;
+        const uint32_t ni = uint32_t(s.blk_int.n), np_ = uint32_t(s.blk_pml.n);
+        const MarchBlock *bi = s.blk_int.p, *bp = s.blk_pml.p;
+#define FG_MARCH(TJ)                                                                                              \
+    do {                                                                                                          \
+        if (ni) hipLaunchKernelGGL((k_march<TJ, false>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (np_) hipLaunchKernelGGL((k_march<TJ, true>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+    } while (0)
         switch (s.tj) {
-            case 1: hipLaunchKernelGGL(k_fused<1>, grid, block, 0, st, g, s.E[a].p, s.E[b].p, s.H[a].p, s.H[b].p, s.psi[a].p, s.psi[b].p, s.seg); break;
-            case 2: hipLaunchKernelGGL(k_fused<2>, grid, block, 0, st, g, s.E[a].p, s.E[b].p, s.H[a].p, s.H[b].p, s.psi[a].p, s.psi[b].p, s.seg); break;
-            case 8: hipLaunchKernelGGL(k_fused<8>, grid, block, 0, st, g, s.E[a].p, s.E[b].p, s.H[a].p, s.H[b].p, s.psi[a].p, s.psi[b].p, s.seg); break;
-            default: hipLaunchKernelGGL(k_fused<4>, grid, block, 0, st, g, s.E[a].p, s.E[b].p, s.H[a].p, s.H[b].p, s.psi[a].p, s.psi[b].p, s.seg); break;
+            case 4: FG_MARCH(4); break;
+            case 14: FG_MARCH(14); break;
+            default: FG_MARCH(8); break;
         }
+#undef FG_MARCH
         Ecur = s.E[b].p;
         Eold = s.E[a].p;
         s.cur = b;

@@ -21,24 +21,40 @@ ENGINES = [abi.FG_ENGINE_FUSED, abi.FG_ENGINE_SPLIT]
 NAMES = {abi.FG_ENGINE_FUSED: "fused", abi.FG_ENGINE_SPLIT: "split"}
 
 
+def peak_relative(rec, ref):
+    """||(dE, eta dH)|| over the run's peak ||(E, eta H)||. The peak comes from the reference's
+    own bookkeeping: decay_db = 10 log10(peak energy / final energy) (run.rs:739-740)."""
+    eta = 376.73
+    final = np.sqrt(np.sum(ref.e.astype(np.float64) ** 2) + np.sum((eta * ref.h.astype(np.float64)) ** 2))
+    scale = final * 10.0 ** (ref.decay_db / 20.0)
+    diff = np.sqrt(np.sum((rec.e.astype(np.float64) - ref.e) ** 2) + np.sum((eta * (rec.h.astype(np.float64) - ref.h)) ** 2))
+    return float(diff / scale)
+
+
 def _compare(rec, o32, o64, what, limit=1e-5):
+    """Integrated outputs (series, plane, ntff) must meet the reference's 1e-5 bar against the
+    FP32 oracle. Pointwise probe series must be no further from the FP64 oracle than twice
+    the FP32 oracle's own distance (plus 1e-6), and within 1e-4 of the FP32 oracle. Final
+    fields must be within 1e-5 of the FP32 oracle relative to the run's peak field norm."""
     print(f"\n{what}: steps {rec.steps}")
     for name in ("series", "plane", "ntff", "probes"):
         a, b = getattr(o32, name), getattr(rec, name)
         if len(a) == 0:
             continue
         w = worst(a, b)
-        w64 = worst(getattr(o64, name), b) if o64 is not None else float("nan")
-        print(f"  {name}: worst vs fp32 oracle {w:.2e}, vs fp64 {w64:.2e}")
-        assert w < limit, f"{what} {name}: {w}"
-        if o64 is not None:
+        w64 = worst(getattr(o64, name), b)
+        ref64 = worst(getattr(o64, name), a)
+        print(f"  {name}: worst vs fp32 oracle {w:.2e}, vs fp64 {w64:.2e} (fp32 oracle vs fp64 {ref64:.2e})")
+        if name == "probes":
+            assert w < 1e-4 and w64 <= 2.0 * ref64 + 1e-6, f"{what} {name}: {w}, {w64}"
+        else:
+            assert w < limit, f"{what} {name}: {w}"
             assert w64 < 1e-4
     if rec.e is not None and o32.e is not None:
-        for name in ("e", "h"):
-            l2 = rel_l2(getattr(o32, name), getattr(rec, name))
-            l64 = rel_l2(getattr(o64, name), getattr(rec, name)) if o64 is not None else float("nan")
-            print(f"  {name}: rel L2 vs fp32 oracle {l2:.2e}, vs fp64 {l64:.2e}")
-            assert l2 < 1e-5, f"{what} {name}: {l2}"
+        d32, d64, ref64 = peak_relative(rec, o32), peak_relative(rec, o64), peak_relative(o32, o64)
+        print(f"  fields: peak-relative L2 vs fp32 oracle {d32:.2e}, vs fp64 {d64:.2e} "
+              f"(fp32 oracle vs fp64 {ref64:.2e})")
+        assert d32 < limit, f"{what} fields: {d32}"
 
 
 @pytest.mark.gpu
@@ -54,6 +70,23 @@ def test_plain_vacuum_matches_the_oracle(gpu, engine):
     assert rec.steps == 500
     assert np.abs(rec.probes).max() > 0
     _compare(rec, o32, o64, f"plain {NAMES[engine]}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
+def test_plain_vacuum_fields_mid_run(gpu, engine):
+    """At 250 steps the pulse is still inside the grid, so E and H are well conditioned:
+    each within 1e-5 relative L2 of the FP32 oracle."""
+    sim, probes = W.plain()
+    ex = F.Extras(max_steps=250, decay_db=math.inf, check_every=50, probes=probes, engine=engine)
+    rec = F.run(gpu, sim, 0, W.plain_pulse(), ex, fields=True)
+    o32 = O.run_cpu(sim.desc, 0, W.plain_pulse(), ex, 32)
+    o64 = O.run_cpu(sim.desc, 0, W.plain_pulse(), ex, 64)
+    for name in ("e", "h"):
+        l32 = rel_l2(getattr(o32, name), getattr(rec, name))
+        l64 = rel_l2(getattr(o64, name), getattr(rec, name))
+        print(f"{name}: rel L2 vs fp32 oracle {l32:.2e}, vs fp64 {l64:.2e}")
+        assert l32 < 1e-5 and l64 < 1e-5
 
 
 @pytest.mark.gpu
