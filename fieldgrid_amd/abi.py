@@ -141,6 +141,7 @@ SIGNATURES = {
     "fg_session_read": (C.c_int, [C.c_void_p, P(fg_record)]),
     "fg_session_free": (None, [C.c_void_p]),
     "fg_device_synchronize": (C.c_int, [C.c_void_p]),
+    "fg_stream_copy": (C.c_int, [C.c_void_p, u64, u32, P(f64)]),
     "fg_sim_new": (C.c_int, [P(fg_sim_desc), P(C.c_void_p)]),
     "fg_sim_free": (None, [C.c_void_p]),
     "fg_sim_dt": (f64, [C.c_void_p]),

@@ -67,6 +67,7 @@ struct Geo {
     uint64_t psi_off[18];  // e*9 + c*3 + a  (run.rs:214-225)
     const uint32_t *coef_index;
     const float2 *coef_sets;
+    uint32_t n_sets;
 };
 
 template <int A>
@@ -208,6 +209,8 @@ __global__ __launch_bounds__(256) void k_update_e(Geo g, float *__restrict__ E, 
 // block writes in the same step. Every node's own E, H, coefficient index and CPML psi are
 // read once and written once per step.
 constexpr int OWN_K = 62;
+constexpr int SETS_LDS = 2048;  // coefficient sets staged in LDS per block (16 KB)
+constexpr int SEG_MAX = 64;     // longest i segment one march block takes
 
 // Block coordinates of one march block (k tile, j tile, i segment). Blocks that touch a CPML
 // slab and blocks that do not are launched as two lists, so the interior kernel carries no
@@ -216,25 +219,64 @@ struct MarchBlock {
     uint32_t bx, by, bz;
 };
 
-// One CPML psi recursion, psi = b*psi + c*diff*raw (cpu.rs:78-99, 137-157), old -> new store.
-__device__ __forceinline__ float cpml(const float *__restrict__ po, float *__restrict__ pn, uint64_t q, float b,
-                                      float c, float raw, float diff, bool store) {
-    const float r = b * po[q] + c * diff * raw;
-    if (store) pn[q] = r;
+// Raw buffer access: a plane-uniform base pointer in a buffer resource (scalar registers)
+// and a 32-bit per-lane byte offset, so no lane carries a 64-bit address. `bytes` bounds the
+// resource: an access past it reads 0 / is dropped by the hardware instead of faulting.
+// 0x00020000 is the gfx9-family (incl. gfx950) resource word 3 for raw dword access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(bytes), 0x00020000);
+}
+__device__ __forceinline__ float ld(const float *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc(p, bytes), int(byte_off), 0, 0));
+}
+__device__ __forceinline__ uint32_t ld(const uint32_t *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rsrc(p, bytes), int(byte_off), 0, 0);
+}
+__device__ __forceinline__ uint16_t ld(const uint16_t *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b16(rsrc(p, bytes), int(byte_off), 0, 0);
+}
+__device__ __forceinline__ void st(float *p, uint32_t byte_off, uint32_t bytes, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc(p, bytes), int(byte_off), 0, 0);
+}
+
+constexpr uint32_t OOB = 0x7FFFFFF0u;  // a byte offset past every resource: load 0, store dropped
+
+// psi = b*old + c*diff*raw (cpu.rs:78-99, 137-157), written to the new store at `off`.
+__device__ __forceinline__ float cpml_next(float *pn, uint32_t bytes, uint32_t off, float b, float c, float raw,
+                                           float old, float diff) {
+    const float r = b * old + c * diff * raw;
+    st(pn, off, bytes, r);
     return r;
 }
 
-template <int TJ, bool PML>
+// One CPML psi recursion, psi = b*psi + c*diff*raw (cpu.rs:78-99, 137-157), from the old store
+// to the new one; po/pn are plane-uniform bases of the psi array, `off` the lane's byte offset.
+__device__ __forceinline__ float cpml(const float *po, float *pn, uint32_t bytes, uint32_t off, float b, float c,
+                                      float raw, float diff, bool store) {
+    const float r = b * ld(po, off, bytes) + c * diff * raw;
+    if (store) st(pn, off, bytes, r);
+    return r;
+}
+
+// PML: 0 = no CPML slab in reach, 1 = only z (k) slabs, 2 = any slab (x, y and z psi code)
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
 __global__ __launch_bounds__(64 * (TJ + 2)) void k_march(Geo g, const MarchBlock *__restrict__ blocks,
                                                          const float *__restrict__ Eo, float *__restrict__ En,
                                                          const float *__restrict__ Ho, float *__restrict__ Hn,
                                                          const float *__restrict__ psio, float *__restrict__ psin,
                                                          uint32_t seg) {
     constexpr int ROWS = TJ + 2;
-    __shared__ float xe[2][ROWS][2][64];  // E(i+1) Ex, Ez of each row, for the row above
-    __shared__ float xh[2][ROWS][2][64];  // new H(i) Hx, Hz of each row, for the row below
+    // row exchanges; rows are stored at index w+1 so that rows -1 and ROWS are zero pads
+    __shared__ float xe[2][ROWS + 2][2][64];  // E(i+1) Ex, Ez of each row, for the row above
+    __shared__ float xh[2][ROWS + 2][2][64];  // new H(i) Hx, Hz of each row, for the row below
+    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    __shared__ float xa[SEG_MAX + 1][PXY ? 10 : 2];  // x-axis values of the segment's planes
     const MarchBlock mb = blocks[blockIdx.x];
-    const int lane = threadIdx.x, w = threadIdx.y;
+    if (mb.bx == ~0u) return;  // padding of the XCD-ordered list (uniform over the block)
+    // blockDim.x is one wave, so the row index is wave-uniform: keep it (and every per-row
+    // value) in scalar registers
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int k = int(mb.bx) * OWN_K + lane - 1;
     const int j = int(mb.by) * TJ + w - 1;
     const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
@@ -248,162 +290,240 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_march(Geo g, const MarchBlock
     const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;  // k+1 / j+1 inside the grid
     const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
     const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
-    // CPML constants of this lane (k) and row (j); psi offsets that do not depend on i
-    int shz = -1, sez = -1, shy = -1, sey = -1;
+    // byte offsets of this lane; a lane that must not touch a plane gets OOB instead, so
+    // every access below is unconditional (no exec-masked branches around memory ops,
+    // which would make the compiler drain all outstanding loads)
+    const uint32_t ob4 = off * 4u;
+    const uint32_t ldE = act ? ob4 : OOB, ldH = act && !bottom ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = own ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    // CPML slots and recursion coefficients of this lane (k) and row (j), and the lane's
+    // loop-invariant byte offsets into the z- and y-slab psi planes
     float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
-    uint32_t ozh = 0, oze = 0, oyh = 0, oye = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
     const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
     if constexpr (PML) {
-        shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
-        shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
         bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
         bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
         bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
         bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
-        ozh = uj * p2 + uint32_t(max(shz, 0)), oze = uj * p2 + uint32_t(max(sez, 0));
-        oyh = uint32_t(max(shy, 0)) * g.n2 + uk, oye = uint32_t(max(sey, 0)) * g.n2 + uk;
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
     }
-    const uint16_t *idx16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    // per-plane (x) axis values staged in LDS: plain loads of them inside the loop would queue
+    // behind the prefetches and force the wave to wait for all of them
     const uint32_t istart = i0 > 0 ? i0 - 1 : 0;
-    float ex = 0.f, ey = 0.f, ez = 0.f;
-    if (act) {
-        const float *p = Eo + istart * plane + off;
-        ex = p[0], ey = p[nn], ez = p[2 * nn];
+    for (uint32_t t = w * 64 + lane; t < i1 - istart; t += 64 * ROWS) {
+        const uint32_t q = istart + t;
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        if constexpr (PXY) {
+            xa[t][2] = g.ax[0].bh[q], xa[t][3] = g.ax[0].ch[q], xa[t][4] = g.ax[0].raw_d[q];
+            xa[t][5] = g.ax[0].be[q], xa[t][6] = g.ax[0].ce[q], xa[t][7] = g.ax[0].raw_dd[q];
+            xa[t][8] = __int_as_float(g.ax[0].slot_h[q]), xa[t][9] = __int_as_float(g.ax[0].slot_e[q]);
+        }
+    }
+    // zero pads (rows -1 and ROWS of both buffers)
+    if (w < 2) {
+        for (int q = 0; q < 2; q++) {
+            xe[q][w * (ROWS + 1)][0][lane] = 0.f, xe[q][w * (ROWS + 1)][1][lane] = 0.f;
+            xh[q][w * (ROWS + 1)][0][lane] = 0.f, xh[q][w * (ROWS + 1)][1][lane] = 0.f;
+        }
+    }
+    // software pipeline: E(i+1), H(i) and the coefficient indices of plane i are loaded one
+    // plane ahead into the other of two register sets; the loop is unrolled by two so the sets
+    // swap roles without copies (a copy would have to wait for the loads in flight)
+    struct Pipe {
+        float ex1, ey1, ez1, hx, hy, hz;
+        uint32_t c0, c1, c2;
+    };
+    Pipe A, B;
+    {
+        const float *p = Eo + uint64_t(istart) * plane;
+        B.ex1 = ld(p, ldE, pl4), B.ey1 = ld(p + nn, ldE, pl4), B.ez1 = ld(p + 2 * nn, ldE, pl4);
     }
     // E(istart) Ex/Ez of the row below, through the buffer the first plane does not write
-    {
-        const int pb = (istart + 1) & 1;
-        xe[pb][w][0][lane] = ex;
-        xe[pb][w][1][lane] = ez;
-    }
+    const int pb0 = (istart + 1) & 1;
+    xe[pb0][w + 1][0][lane] = B.ex1;
+    xe[pb0][w + 1][1][lane] = B.ez1;
     __syncthreads();
-    float exn = 0.f, ezn = 0.f;
-    if (w + 1 < ROWS) {
-        const int pb = (istart + 1) & 1;
-        exn = xe[pb][w + 1][0][lane];
-        ezn = xe[pb][w + 1][1][lane];
+    float exn = xe[pb0][w + 2][0][lane], ezn = xe[pb0][w + 2][1][lane];
+    {
+        const uint64_t pb = uint64_t(istart) * plane;
+        const uint32_t eb = istart + 1 < g.n0 ? pl4 : 0u;  // a zero-size resource reads 0
+        const float *p = Eo + pb + (istart + 1 < g.n0 ? plane : 0);
+        A.ex1 = ld(p, ldE, eb), A.ey1 = ld(p + nn, ldE, eb), A.ez1 = ld(p + 2 * nn, ldE, eb);
+        const float *q = Ho + pb;
+        A.hx = ld(q, ldH, pl4), A.hy = ld(q + nn, ldH, pl4), A.hz = ld(q + 2 * nn, ldH, pl4);
+        A.c0 = cidx(pb, plc), A.c1 = cidx(nn + pb, plc), A.c2 = cidx(2 * nn + pb, plc);
     }
     float hy_p = 0.f, hz_p = 0.f;
-    for (uint32_t i = istart; i < i1; i++) {
-        const int b = i & 1;
-        const bool write = own && i >= i0;
-        const bool il = i + 1 < g.n0;
-        const uint64_t base = i * plane + off;
-        float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f;
-        if (act && il) {
-            const float *p = Eo + base + plane;
-            ex1 = p[0], ey1 = p[nn], ez1 = p[2 * nn];
-        }
-        const float idx_ = g.ax[0].inv_d[i], iddx = g.ax[0].inv_dd[i];
-        // CPML values of plane i and the psi bases of this plane
-        int shx = -1, sex = -1;
-        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
-        uint64_t zrow = 0, yrow = 0, xho = 0, xeo = 0;
-        if constexpr (PML) {
-            shx = g.ax[0].slot_h[i], sex = g.ax[0].slot_e[i];
-            bhx = g.ax[0].bh[i], chx = g.ax[0].ch[i], rdx = g.ax[0].raw_d[i];
-            bex = g.ax[0].be[i], cex = g.ax[0].ce[i], rddx = g.ax[0].raw_dd[i];
-            zrow = uint64_t(i) * g.n1 * p2;             // a = 2: (i*n1 + j)*2p + s
-            yrow = uint64_t(i) * p2 * g.n2;             // a = 1: (i*2p + s)*n2 + k
-            xho = uint64_t(max(shx, 0)) * plane + off;  // a = 0: (s*n1 + j)*n2 + k
-            xeo = uint64_t(max(sex, 0)) * plane + off;
-        }
 #define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    auto plane_step = [&](uint32_t i, Pipe &cur, Pipe &nxt) {
+        const int b = i & 1;
+        const bool wplane = i >= i0;
+        const bool il = i + 1 < g.n0;
+        const uint64_t pb = uint64_t(i) * plane;  // uniform
+        // E(i) arrived two planes ago in the set this plane's prefetch now refills
+        const float ex = nxt.ex1, ey = nxt.ey1, ez = nxt.ez1;
+        // prefetch of plane i+1 (E of plane i+2); absent planes read through zero-size resources
+        const bool more = i + 1 < i1, more2 = more && i + 2 < g.n0;
+        const float *pe = Eo + pb + (more2 ? 2 * plane : 0);
+        nxt.ex1 = ld(pe, ldE, more2 ? pl4 : 0u), nxt.ey1 = ld(pe + nn, ldE, more2 ? pl4 : 0u),
+        nxt.ez1 = ld(pe + 2 * nn, ldE, more2 ? pl4 : 0u);
+        const float *ph = Ho + pb + (more ? plane : 0);
+        nxt.hx = ld(ph, ldH, more ? pl4 : 0u), nxt.hy = ld(ph + nn, ldH, more ? pl4 : 0u),
+        nxt.hz = ld(ph + 2 * nn, ldH, more ? pl4 : 0u);
+        const uint64_t cb_ = pb + (more ? plane : 0);
+        nxt.c0 = cidx(cb_, more ? plc : 0u), nxt.c1 = cidx(nn + cb_, more ? plc : 0u),
+        nxt.c2 = cidx(2 * nn + cb_, more ? plc : 0u);
+        const float *xv = xa[i - istart];
+        const float idx_ = xv[0], iddx = xv[1];
+        // CPML: every psi value this lane needs on plane i is loaded here, up front, so the
+        // loads overlap. A slot that does not exist maps to an offset past its resource (the
+        // load returns 0 and the store is dropped) and has b = c = 0, so its term adds exactly 0.
+        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
+        float hxy = 0, hxz = 0, hyz = 0, hyx = 0, hzx = 0, hzy = 0;  // old H psi (component, axis)
+        float exy = 0, exz = 0, eyz = 0, eyx = 0, ezx = 0, ezy = 0;  // old E psi
+        uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
+        uint32_t oxh = OOB, oxe = OOB;
+        if constexpr (PZ) {
+            zrow = uint64_t(i) * g.n1 * p2;  // a = 2: (i*n1 + j)*2p + s
+            const uint32_t zh = ldH == OOB ? OOB : ozh, ze = stO == OOB ? OOB : oze;
+            hxz = ld(psio + PSI(1, 0, 2) + zrow, zh, zbytes), hyz = ld(psio + PSI(1, 1, 2) + zrow, zh, zbytes);
+            exz = ld(psio + PSI(0, 0, 2) + zrow, ze, zbytes), eyz = ld(psio + PSI(0, 1, 2) + zrow, ze, zbytes);
+        }
+        if constexpr (PXY) {
+            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
+            bhx = xv[2], chx = xv[3], rdx = xv[4];
+            bex = xv[5], cex = xv[6], rddx = xv[7];
+            yrow = uint64_t(i) * p2 * g.n2;  // a = 1: (i*2p + s)*n2 + k
+            xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;  // a = 0
+            oxh = shx >= 0 ? ldE : OOB, oxe = sex >= 0 ? stO : OOB;
+            const uint32_t yh = ldH == OOB ? OOB : oyh, xh_ = ldH == OOB ? OOB : oxh;
+            hxy = ld(psio + PSI(1, 0, 1) + yrow, yh, ybytes);
+            hyx = ld(psio + PSI(1, 1, 0) + xhb, xh_, pl4);
+            hzx = ld(psio + PSI(1, 2, 0) + xhb, xh_, pl4), hzy = ld(psio + PSI(1, 2, 1) + yrow, yh, ybytes);
+            const uint32_t ye = stO == OOB ? OOB : oye;
+            exy = ld(psio + PSI(0, 0, 1) + yrow, ye, ybytes);
+            eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, pl4);
+            ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, pl4), ezy = ld(psio + PSI(0, 2, 1) + yrow, ye, ybytes);
+        }
+        // stores of this plane: only owners, only on planes of their own segment
+        const uint32_t sto = wplane ? stO : OOB;
+        // new psi from its old value; stored (where the slot exists) only by the node's owner
+#define CP(base, bytes, o, B, CC, R, old, diff) cpml_next(psin + (base), bytes, sto == OOB ? OOB : (o), B, CC, R, old, diff)
         // k+1 neighbours of E(i) from the next lane
         const float ex_k = __shfl_down(ex, 1, 64), ey_k = __shfl_down(ey, 1, 64);
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        if (act && !bottom) {
-            const float *ph = Ho + base;
-            const float hxo = ph[0], hyo = ph[nn], hzo = ph[2 * nn];
+        float hx, hy, hz;
+        {
             // Hx: u = y, v = z
-            hx = hxo;
-            if (jl && kl) {
-                const float ev = ezn - ez, eu = ey_k - ey;
-                float t1 = ev * idy, t2 = eu * idz;
-                if (PML && shy >= 0) t1 += cpml(psio, psin, PSI(1, 0, 1) + yrow + oyh, bhy, chy, rdy, ev, write);
-                if (PML && shz >= 0) t2 += cpml(psio, psin, PSI(1, 0, 2) + zrow + ozh, bhz, chz, rdz, eu, write);
-                hx = hxo - g.k_mu * (t1 - t2);
-            }
-            // Hy: u = z, v = x
-            hy = hyo;
-            if (kl && il) {
-                const float ev = ex_k - ex, eu = ez1 - ez;
-                float t1 = ev * idz, t2 = eu * idx_;
-                if (PML && shz >= 0) t1 += cpml(psio, psin, PSI(1, 1, 2) + zrow + ozh, bhz, chz, rdz, ev, write);
-                if (PML && shx >= 0) t2 += cpml(psio, psin, PSI(1, 1, 0) + xho, bhx, chx, rdx, eu, write);
-                hy = hyo - g.k_mu * (t1 - t2);
-            }
-            // Hz: u = x, v = y
-            hz = hzo;
-            if (il && jl) {
-                const float ev = ey1 - ey, eu = exn - ex;
-                float t1 = ev * idx_, t2 = eu * idy;
-                if (PML && shx >= 0) t1 += cpml(psio, psin, PSI(1, 2, 0) + xho, bhx, chx, rdx, ev, write);
-                if (PML && shy >= 0) t2 += cpml(psio, psin, PSI(1, 2, 1) + yrow + oyh, bhy, chy, rdy, eu, write);
-                hz = hzo - g.k_mu * (t1 - t2);
-            }
-            if (write) {
-                float *q = Hn + base;
-                q[0] = hx, q[nn] = hy, q[2 * nn] = hz;
-            }
+            const float ev = ezn - ez, eu = ey_k - ey;
+            float t1 = ev * idy, t2 = eu * idz;
+            const bool v = jl && kl;  // psi advances only where the H component does
+            if constexpr (PXY) t1 += CP(PSI(1, 0, 1) + yrow, ybytes, v ? oyh : OOB, bhy, chy, rdy, hxy, ev);
+            if constexpr (PZ) t2 += CP(PSI(1, 0, 2) + zrow, zbytes, v ? ozh : OOB, bhz, chz, rdz, hxz, eu);
+            hx = jl && kl ? cur.hx - g.k_mu * (t1 - t2) : cur.hx;
         }
-        xh[b][w][0][lane] = hx;
-        xh[b][w][1][lane] = hz;
-        xe[b][w][0][lane] = ex1;
-        xe[b][w][1][lane] = ez1;
+        {
+            // Hy: u = z, v = x
+            const float ev = ex_k - ex, eu = cur.ez1 - ez;
+            float t1 = ev * idz, t2 = eu * idx_;
+            const bool v = kl && il;
+            if constexpr (PZ) t1 += CP(PSI(1, 1, 2) + zrow, zbytes, v ? ozh : OOB, bhz, chz, rdz, hyz, ev);
+            if constexpr (PXY) t2 += CP(PSI(1, 1, 0) + xhb, pl4, v ? oxh : OOB, bhx, chx, rdx, hyx, eu);
+            hy = kl && il ? cur.hy - g.k_mu * (t1 - t2) : cur.hy;
+        }
+        {
+            // Hz: u = x, v = y
+            const float ev = cur.ey1 - ey, eu = exn - ex;
+            float t1 = ev * idx_, t2 = eu * idy;
+            const bool v = il && jl;
+            if constexpr (PXY) t1 += CP(PSI(1, 2, 0) + xhb, pl4, v ? oxh : OOB, bhx, chx, rdx, hzx, ev);
+            if constexpr (PXY) t2 += CP(PSI(1, 2, 1) + yrow, ybytes, v ? oyh : OOB, bhy, chy, rdy, hzy, eu);
+            hz = il && jl ? cur.hz - g.k_mu * (t1 - t2) : cur.hz;
+        }
+        {
+            float *q = Hn + pb;
+            st(q, sto, pl4, hx), st(q + nn, sto, pl4, hy), st(q + 2 * nn, sto, pl4, hz);
+        }
+        xh[b][w + 1][0][lane] = hx;
+        xh[b][w + 1][1][lane] = hz;
+        xe[b][w + 1][0][lane] = cur.ex1;
+        xe[b][w + 1][1][lane] = cur.ez1;
         __syncthreads();
-        const float hx_j = w > 0 ? xh[b][w - 1][0][lane] : 0.f, hz_j = w > 0 ? xh[b][w - 1][1][lane] : 0.f;
-        const float exn1 = w + 1 < ROWS ? xe[b][w + 1][0][lane] : 0.f;
-        const float ezn1 = w + 1 < ROWS ? xe[b][w + 1][1][lane] : 0.f;
+        const float hx_j = xh[b][w][0][lane], hz_j = xh[b][w][1][lane];
+        const float exn1 = xe[b][w + 2][0][lane], ezn1 = xe[b][w + 2][1][lane];
         // k-1 neighbours of new H(i) from the previous lane
         const float hx_k = __shfl_up(hx, 1, 64), hy_k = __shfl_up(hy, 1, 64);
-        if (write) {
-            float *q = En + base;
+        {
+            float *q = En + pb;
             // Ex: u = y, v = z
             {
-                const float2 cf = g.coef_sets[g.wide ? g.coef_index[base] : idx16[base]];
-                float e = ex;
-                if (cf.y != 0.f) {
-                    const float hv = hz - hz_j, hu = hy - hy_k;
-                    float t1 = hv * iddy, t2 = hu * iddz;
-                    if (PML && sey >= 0) t1 += cpml(psio, psin, PSI(0, 0, 1) + yrow + oye, bey, cey, rddy, hv, true);
-                    if (PML && sez >= 0) t2 += cpml(psio, psin, PSI(0, 0, 2) + zrow + oze, bez, cez, rddz, hu, true);
-                    e = cf.x * ex + cf.y * (t1 - t2);
-                }
-                q[0] = e;
+                const float2 cf = coef(cur.c0);
+                const float hv = hz - hz_j, hu = hy - hy_k;
+                float t1 = hv * iddy, t2 = hu * iddz;
+                const bool on = cf.y != 0.f;
+                if constexpr (PXY) t1 += CP(PSI(0, 0, 1) + yrow, ybytes, on ? oye : OOB, bey, cey, rddy, exy, hv);
+                if constexpr (PZ) t2 += CP(PSI(0, 0, 2) + zrow, zbytes, on ? oze : OOB, bez, cez, rddz, exz, hu);
+                st(q, sto, pl4, cf.y != 0.f ? cf.x * ex + cf.y * (t1 - t2) : ex);
             }
             // Ey: u = z, v = x
             {
-                const float2 cf = g.coef_sets[g.wide ? g.coef_index[nn + base] : idx16[nn + base]];
-                float e = ey;
-                if (cf.y != 0.f) {
-                    const float hv = hx - hx_k, hu = hz - hz_p;
-                    float t1 = hv * iddz, t2 = hu * iddx;
-                    if (PML && sez >= 0) t1 += cpml(psio, psin, PSI(0, 1, 2) + zrow + oze, bez, cez, rddz, hv, true);
-                    if (PML && sex >= 0) t2 += cpml(psio, psin, PSI(0, 1, 0) + xeo, bex, cex, rddx, hu, true);
-                    e = cf.x * ey + cf.y * (t1 - t2);
-                }
-                q[nn] = e;
+                const float2 cf = coef(cur.c1);
+                const float hv = hx - hx_k, hu = hz - hz_p;
+                float t1 = hv * iddz, t2 = hu * iddx;
+                const bool on = cf.y != 0.f;
+                if constexpr (PZ) t1 += CP(PSI(0, 1, 2) + zrow, zbytes, on ? oze : OOB, bez, cez, rddz, eyz, hv);
+                if constexpr (PXY) t2 += CP(PSI(0, 1, 0) + xeb, pl4, on ? oxe : OOB, bex, cex, rddx, eyx, hu);
+                st(q + nn, sto, pl4, cf.y != 0.f ? cf.x * ey + cf.y * (t1 - t2) : ey);
             }
             // Ez: u = x, v = y
             {
-                const float2 cf = g.coef_sets[g.wide ? g.coef_index[2 * nn + base] : idx16[2 * nn + base]];
-                float e = ez;
-                if (cf.y != 0.f) {
-                    const float hv = hy - hy_p, hu = hx - hx_j;
-                    float t1 = hv * iddx, t2 = hu * iddy;
-                    if (PML && sex >= 0) t1 += cpml(psio, psin, PSI(0, 2, 0) + xeo, bex, cex, rddx, hv, true);
-                    if (PML && sey >= 0) t2 += cpml(psio, psin, PSI(0, 2, 1) + yrow + oye, bey, cey, rddy, hu, true);
-                    e = cf.x * ez + cf.y * (t1 - t2);
-                }
-                q[2 * nn] = e;
+                const float2 cf = coef(cur.c2);
+                const float hv = hy - hy_p, hu = hx - hx_j;
+                float t1 = hv * iddx, t2 = hu * iddy;
+                const bool on = cf.y != 0.f;
+                if constexpr (PXY) t1 += CP(PSI(0, 2, 0) + xeb, pl4, on ? oxe : OOB, bex, cex, rddx, ezx, hv);
+                if constexpr (PXY) t2 += CP(PSI(0, 2, 1) + yrow, ybytes, on ? oye : OOB, bey, cey, rddy, ezy, hu);
+                st(q + 2 * nn, sto, pl4, cf.y != 0.f ? cf.x * ez + cf.y * (t1 - t2) : ez);
             }
         }
-#undef PSI
-        ex = ex1, ey = ey1, ez = ez1;
+#undef CP
         exn = exn1, ezn = ezn1;
         hy_p = hy, hz_p = hz;
+    };
+    // both unrolled copies are unconditional (an odd plane count is peeled first) so the
+    // compiler keeps them apart instead of merging them back into one body with copies
+    if ((i1 - istart) & 1u) {
+        plane_step(istart, A, B);
+        for (uint32_t i = istart + 1; i < i1; i += 2) {
+            plane_step(i, B, A);
+            plane_step(i + 1, A, B);
+        }
+    } else {
+        for (uint32_t i = istart; i < i1; i += 2) {
+            plane_step(i, A, B);
+            plane_step(i + 1, B, A);
+        }
     }
+#undef PSI
 }
 
 // ---- sparse per-step fixes: Debye ADE, port sources, inductors (cpu.rs:239-303) ----------
@@ -612,6 +732,14 @@ __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ E, con
 
 __global__ void k_set_step(uint32_t *step, uint32_t value) { step[0] = value; }
 
+// Plain streaming copy, 16 B per lane per access: the bandwidth probe that calibrates the
+// roofline and the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                     uint64_t count) {
+    for (uint64_t q = uint64_t(blockIdx.x) * 256 + threadIdx.x; q < count; q += uint64_t(gridDim.x) * 256)
+        dst[q] = src[q];
+}
+
 // ---- host side ------------------------------------------------------------------------------
 
 template <class T>
@@ -659,7 +787,7 @@ struct Session {
     uint64_t nn = 0;
     uint32_t n[3];
     bool fused = true;
-    uint32_t tj = 4, seg = 8;
+    uint32_t tj = 14, seg = 32, xcds = 8;
     uint64_t max_steps = 0, steps = 0;
     uint32_t n_ports = 0, n_probes = 0, nf = 0;
     uint64_t plane_len = 0, ntff_len = 0;
@@ -683,7 +811,7 @@ struct Session {
     DevBuf<uint64_t> probe_f;
     DevBuf<uint32_t> patch;
     DevBuf<double> partial;
-    DevBuf<MarchBlock> blk_int, blk_pml;  // march blocks clear of / touching the CPML slabs
+    DevBuf<MarchBlock> blk_int, blk_z, blk_pml;  // march blocks clear of CPML / touching z slabs only / any
     std::vector<hipEvent_t> events;
 
     ~Session() {
@@ -753,16 +881,34 @@ static void plan_march(Session &s, const fg_problem *P, const std::vector<int> &
         return hi >= lo && pre[a][size_t(hi) + 1] > pre[a][size_t(lo)];
     };
     const uint32_t gx = (P->n[2] + OWN_K - 1) / OWN_K, gy = (P->n[1] + s.tj - 1) / s.tj, gz = (P->n[0] + s.seg - 1) / s.seg;
-    std::vector<MarchBlock> in, pm;
+    std::vector<MarchBlock> in, zs, pm;
     for (uint32_t bz = 0; bz < gz; bz++)
         for (uint32_t by = 0; by < gy; by++)
             for (uint32_t bx = 0; bx < gx; bx++) {
                 const int64_t k0 = int64_t(bx) * OWN_K, j0 = int64_t(by) * s.tj, i0 = int64_t(bz) * s.seg;
-                const bool slab = touches(2, k0 - 1, k0 + OWN_K) || touches(1, j0 - 1, j0 + s.tj) ||
-                                  touches(0, i0 - 1, i0 + s.seg);
-                (slab ? pm : in).push_back(MarchBlock{bx, by, bz});
+                const bool z = touches(2, k0 - 1, k0 + OWN_K);
+                const bool xy = touches(1, j0 - 1, j0 + s.tj) || touches(0, i0 - 1, i0 + s.seg);
+                (xy ? pm : z ? zs : in).push_back(MarchBlock{bx, by, bz});
             }
+    // Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute each
+    // list so that XCD x receives a contiguous run of the logical order (k tiles fastest, then
+    // j tiles): neighbouring tiles then share one L2, and the cache lines their halos and
+    // unaligned edges have in common are fetched from HBM once.
+    auto xcd_order = [&](std::vector<MarchBlock> &v) {
+        const size_t N = v.size(), X = s.xcds, per = (N + X - 1) / X;
+        if (X <= 1 || N == 0) return;
+        std::vector<MarchBlock> out(per * X, MarchBlock{~0u, ~0u, ~0u});
+        for (size_t b = 0; b < per * X; b++) {
+            const size_t L = (b % X) * per + b / X;
+            if (L < N) out[b] = v[L];
+        }
+        v.swap(out);
+    };
+    xcd_order(in);
+    xcd_order(zs);
+    xcd_order(pm);
     s.blk_int.upload(in.data(), in.size(), s.stream);
+    s.blk_z.upload(zs.data(), zs.size(), s.stream);
     s.blk_pml.upload(pm.data(), pm.size(), s.stream);
 }
 
@@ -833,6 +979,7 @@ static Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven
     for (int q = 0; q < 18; q++) g.psi_off[q] = off[q];
     g.coef_index = s.coef_index.p;
     g.coef_sets = reinterpret_cast<const float2 *>(s.coef_sets.p);
+    g.n_sets = P->n_sets;
     // sparse fixes
     s.max_steps = X->max_steps;
     s.step.alloc(1);
@@ -932,11 +1079,14 @@ static Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven
     s.obs_blocks = o.blk_ports + o.blk_probes + o.blk_plane + o.blk_ntff;
     s.partial.alloc(ENERGY_BLOCKS);
     // fused march: TJ owned rows (+2 halo rows) x 62 owned lanes, `seg` planes of i per block
-    s.tj = 8;
-    s.seg = 64;
+    s.tj = 14;
+    s.seg = 32;
+    s.xcds = dev->cus >= 64 ? uint32_t(dev->cus / 32) : 1u;  // gfx950: 32 CUs per XCD
     if (const char *v = getenv("FIELDGRID_TJ")) s.tj = uint32_t(atoi(v));
     if (const char *v = getenv("FIELDGRID_SEG")) s.seg = std::max(1u, uint32_t(atoi(v)));
-    if (s.tj != 4 && s.tj != 8 && s.tj != 14) s.tj = 8;
+    if (const char *v = getenv("FIELDGRID_XCDS")) s.xcds = std::max(1u, uint32_t(atoi(v)));
+    if (s.tj != 8 && s.tj != 14) s.tj = 14;
+    s.seg = std::min<uint32_t>(s.seg, SEG_MAX);
     if (s.fused) plan_march(s, P, axs, nmax);
     FG_CHECK(hipStreamSynchronize(st));
     return S.release();
@@ -979,18 +1129,28 @@ static void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
         dim3 block(64, s.tj + 2);
         const float *Eo = s.E[a].p, *Ho = s.H[a].p, *Po = s.psi[a].p;
         float *En = s.E[b].p, *Hn = s.H[b].p, *Pn = s.psi[b].p;
-        const uint32_t ni = uint32_t(s.blk_int.n), np_ = uint32_t(s.blk_pml.n);
-        const MarchBlock *bi = s.blk_int.p, *bp = s.blk_pml.p;
-#define FG_MARCH(TJ)                                                                                              \
+        const uint32_t ni = uint32_t(s.blk_int.n), nz = uint32_t(s.blk_z.n), np_ = uint32_t(s.blk_pml.n);
+        const MarchBlock *bi = s.blk_int.p, *bz = s.blk_z.p, *bp = s.blk_pml.p;
+#define FG_MARCH2(TJ, W, L)                                                                                       \
     do {                                                                                                          \
-        if (ni) hipLaunchKernelGGL((k_march<TJ, false>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg); \
-        if (np_) hipLaunchKernelGGL((k_march<TJ, true>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (ni) hipLaunchKernelGGL((k_march<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (nz) hipLaunchKernelGGL((k_march<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (np_) hipLaunchKernelGGL((k_march<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+    } while (0)
+#define FG_MARCH(TJ)                                                   \
+    do {                                                               \
+        const bool lds = g.n_sets <= uint32_t(SETS_LDS);               \
+        if (g.wide) {                                                  \
+            if (lds) FG_MARCH2(TJ, true, true); else FG_MARCH2(TJ, true, false);   \
+        } else {                                                       \
+            if (lds) FG_MARCH2(TJ, false, true); else FG_MARCH2(TJ, false, false); \
+        }                                                              \
     } while (0)
         switch (s.tj) {
-            case 4: FG_MARCH(4); break;
-            case 14: FG_MARCH(14); break;
-            default: FG_MARCH(8); break;
+            case 8: FG_MARCH(8); break;
+            default: FG_MARCH(14); break;
         }
+#undef FG_MARCH2
 #undef FG_MARCH
         Ecur = s.E[b].p;
         Eold = s.E[a].p;
@@ -1169,6 +1329,36 @@ int fg_session_timing(fg_session *ss, int enable, double ms[3], uint64_t *steps)
 int fg_session_sync(fg_session *ss) {
     return fg::guarded([&] {
         FG_CHECK(hipStreamSynchronize(reinterpret_cast<fg::Session *>(ss)->stream));
+        return int(FG_OK);
+    });
+}
+
+int fg_stream_copy(fg_device *dev, uint64_t bytes, uint32_t iterations, double *gbps) {
+    return fg::guarded([&] {
+        FG_CHECK(hipSetDevice(dev->ordinal));
+        const uint64_t count = bytes / 16;
+        fg::DevBuf<float4> a, b;
+        a.alloc(count);
+        b.alloc(count);
+        hipStream_t st;
+        FG_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        a.zero(st);
+        const uint32_t blocks = uint32_t(std::min<uint64_t>((count + 255) / 256, uint64_t(dev->cus) * 64));
+        hipEvent_t e0, e1;
+        FG_CHECK(hipEventCreate(&e0));
+        FG_CHECK(hipEventCreate(&e1));
+        hipLaunchKernelGGL(fg::k_stream_copy, dim3(blocks), dim3(256), 0, st, a.p, b.p, count);  // warm-up
+        FG_CHECK(hipEventRecord(e0, st));
+        for (uint32_t q = 0; q < iterations; q++)
+            hipLaunchKernelGGL(fg::k_stream_copy, dim3(blocks), dim3(256), 0, st, a.p, b.p, count);
+        FG_CHECK(hipEventRecord(e1, st));
+        FG_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FG_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+        if (gbps) *gbps = 2.0 * double(count) * 16.0 * iterations / (double(ms) * 1e-3) / 1e9;
         return int(FG_OK);
     });
 }

@@ -461,6 +461,12 @@ class Gpu:
     def synchronize(self):
         abi.check(abi.lib().fg_device_synchronize(self._h))
 
+    def stream_copy(self, nbytes: int = 1 << 30, iterations: int = 20) -> float:
+        """Achieved read+write GB/s of a plain device copy (the bandwidth probe)."""
+        v = C.c_double()
+        abi.check(abi.lib().fg_stream_copy(self._h, nbytes, iterations, C.byref(v)))
+        return v.value
+
     def __del__(self):
         if getattr(self, "_h", None):
             abi.lib().fg_close(self._h)
