@@ -79,15 +79,13 @@ def cpu_baseline(name, budget_s=20.0):
         extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf, probes=probes)
     n = sim.dims()
     nn = n[0] * n[1] * n[2]
-    t = time.perf_counter()
-    O.run_cpu(sim.desc, 0, pulse, extras(2), 32, fields=False)  # warm-up and size probe
-    per_step = max((time.perf_counter() - t) / 2, 1e-6)
+    probe = O.run_cpu(sim.desc, 0, pulse, extras(2), 32, fields=False)  # warm-up and size probe
+    per_step = max(probe.step_seconds / 2, 1e-6)
     k = max(2, min(2000, int(budget_s / per_step)))
-    t = time.perf_counter()
     r = O.run_cpu(sim.desc, 0, pulse, extras(k), 32, fields=False)
-    dt = time.perf_counter() - t
+    dt = r.step_seconds  # the stepping loop alone, as on the GPU side
     return {"value": round(nn * r.steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": O.threads(),
-            "kind": "port", "sample": f"{sample}: {r.steps} steps, {nn} nodes, {dt:.1f} s"}
+            "kind": "port", "sample": f"{sample}: {r.steps} steps of {nn} nodes in {dt:.1f} s"}
 
 
 def main():
@@ -141,6 +139,12 @@ def main():
     # algorithmic bytes of the dominant (sweep) kernel per launch: E/H/coef/psi traffic only
     core_bpn = W.bytes_per_node_step(sim, None, core=True)
     achieved = core_bpn * nn / (sweep_ms * 1e-3) / 1e9
+    # HBM traffic per sweep from the committed rocprofv3 PMC summary of this workload
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"k_march_{args.workload}_traffic.json")
+    if args.engine == "fused" and os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("traffic_bytes_per_step")
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(secs * 1e3 / args.steps, 4),
@@ -150,8 +154,10 @@ def main():
                        hbm_frac_whole_step=round(bpn * nn * args.steps / secs / 1e9 / HBM_PEAK_GBS, 4),
                        build_s=round(build_s, 2), gpu=gpu.name),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_fused" if args.engine == "fused" else "k_update_h+k_update_e",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": round(core_bpn * nn),
+                     "kernel": ("k_march sweep (interior, z-CPML, CPML launches)" if args.engine == "fused"
+                                else "k_update_h + k_update_e"),
                      "kernel_ms": round(sweep_ms, 4), "alg_bytes_per_node": round(core_bpn, 3),
                      "fix_ms": round(ms[1] / max(timed, 1), 4), "observe_ms": round(ms[2] / max(timed, 1), 4)},
     }

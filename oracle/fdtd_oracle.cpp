@@ -26,6 +26,7 @@
 // dipole NTFF directivity and radiated power).
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1212,6 +1213,7 @@ typedef struct oracle_out {
     double *e_out;
     double *h_out;
     double dt;
+    double step_seconds;  // wall time of the stepping loop alone (no build, no pack)
 } oracle_out;
 
 }
@@ -1232,6 +1234,7 @@ static int run_impl(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
     double peak = 0.0, decay = 0.0;
     uint64_t calls = 0;
     int status = FG_OK;
+    const auto t_start = std::chrono::steady_clock::now();
     while (steps < ex->max_steps) {
         size_t k = std::min<size_t>(chunk, ex->max_steps - steps);
         for (size_t s = 0; s < k; s++) st.step(steps + s);
@@ -1252,6 +1255,7 @@ static int run_impl(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
         }
         if (steps > std::max(pulse_end, size_t(ex->min_steps)) && decay >= ex->decay_db) break;
     }
+    out->step_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     out->steps = steps;
     out->decay_db = decay;
     out->progress_calls = calls;

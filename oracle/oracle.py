@@ -20,7 +20,8 @@ class oracle_out(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("decay_db", C.c_double), ("progress_calls", C.c_uint64),
                 ("series", C.POINTER(C.c_double)), ("plane", C.POINTER(C.c_double)),
                 ("ntff", C.POINTER(C.c_double)), ("probes", C.POINTER(C.c_double)),
-                ("e_out", C.POINTER(C.c_double)), ("h_out", C.POINTER(C.c_double)), ("dt", C.c_double)]
+                ("e_out", C.POINTER(C.c_double)), ("h_out", C.POINTER(C.c_double)), ("dt", C.c_double),
+                ("step_seconds", C.c_double)]
 
 
 _lib = None
@@ -78,6 +79,7 @@ class OracleRecord:
     e: Optional[np.ndarray]
     h: Optional[np.ndarray]
     dt: float
+    step_seconds: float = 0.0
 
 
 def _dptr(a):
@@ -115,7 +117,8 @@ def run_cpu(desc, driven, pulse, extras, precision=32, cancel_at=0, dt=None, fie
     plane_len = n[0] * n[1] * nf * 10 if extras.plane_k is not None else 0
     return OracleRecord(steps, out.decay_db, int(out.progress_calls), bufs["series"][: steps * ports * 2],
                         bufs["plane"][:plane_len], bufs["ntff"][: int(x.n_patches) * nf * 12],
-                        bufs["probes"][: steps * len(extras.probes)], bufs.get("e"), bufs.get("h"), out.dt)
+                        bufs["probes"][: steps * len(extras.probes)], bufs.get("e"), bufs.get("h"), out.dt,
+                        out.step_seconds)
 
 
 class Packed:
