@@ -1,6 +1,6 @@
 """FDTD throughput benchmark on one MI355X: Mcell-updates/s and % of the HBM roofline.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload layered|boxes|plain]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload layered|boxes|plain|sweep]
 
 A step is one FDTD time step (H and E curl updates with CPML, sparse fixes, port/probe
 observers) over the whole grid. The default workload is the single large throughput run:
@@ -8,9 +8,15 @@ observers) over the whole grid. The default workload is the single large through
 50-ohm z-ports. Inputs are resident in HBM before the timed region starts. Prints one JSON
 line with `roofline` (dominant kernel, HIP-event timed) and `cpu_baseline` (the oracle's
 FP32 restatement of the reference CPU path on a bounded sample of the same recipe).
+
+--workload sweep measures the batched sweep instead: 96 seeded runs of uneven grids (48-200
+nodes per axis, CPML 8, 1-4 ports, 800-3000 steps each), first one after another through the
+single-run engine, then together through fg_run_batch (autotuning included), and reports
+whole-job Mcell-updates/s of the batch and its speedup over the sequential runs.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -36,6 +42,12 @@ def build_workload(name):
         extras_kw = {"boxes": True}
         config = {"workload": "boxes + Debye + port + NTFF", "dims": [256, 256, 256], "dx_mm": 0.5, "pml": 12,
                   "boxes": 40, "debye_boxes": 10, "ntff_freqs_ghz": [2, 5, 10]}
+    elif name == "sweep":
+        runs = W.sweep_runs()
+        r0 = min(runs, key=lambda r: np.prod(r.sim.dims()))
+        sim, pulse = r0.sim, r0.pulse
+        sample = f"the smallest run of the sweep ({'x'.join(map(str, sim.dims()))})"
+        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf)
     elif name == "plain":
         sim, probes = W.plain()
         pulse = W.plain_pulse()
@@ -59,6 +71,7 @@ def cpu_baseline(name, budget_s=20.0):
     """The oracle's FP32 restatement of run_cpu (cpu.rs:424-482) on the host cores, on a
     bounded sample of the same recipe. Returns the dict for the JSON line."""
     import math
+    import numpy as np
     from fieldgrid_amd import fdtd as F
     from fieldgrid_amd import workloads as W
     from oracle import oracle as O
@@ -88,12 +101,69 @@ def cpu_baseline(name, budget_s=20.0):
             "kind": "port", "sample": f"{sample}: {r.steps} steps of {nn} nodes in {dt:.1f} s"}
 
 
+def sweep(args):
+    """The batched sweep: sequential single runs, then one fg_run_batch of the same runs."""
+    import numpy as np
+    from fieldgrid_amd import fdtd as F
+    from fieldgrid_amd import workloads as W
+    t_build = time.perf_counter()
+    runs = W.sweep_runs()
+    build_s = time.perf_counter() - t_build
+    node_steps = sum(int(np.prod(r.sim.dims())) * r.extras.max_steps for r in runs)
+    for r in runs:
+        r.sim.problem()  # pack every problem before either timed region
+    gpu = F.Gpu(0)
+    # warm-up: one small run and one small batch load the kernels
+    F.run(gpu, runs[0].sim, runs[0].driven, runs[0].pulse, F.Extras(max_steps=4, decay_db=math.inf))
+    F.run_batch(gpu, runs[:2], tune=F.Tune(tj=14, seg=16, chunk=1))
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    seq = [F.run(gpu, r.sim, r.driven, r.pulse, r.extras) for r in runs]
+    gpu.synchronize()
+    seq_s = time.perf_counter() - t0
+    tune = F.Tune()
+    t0 = time.perf_counter()
+    recs, spectra = F.run_batch(gpu, runs, tune=tune)
+    gpu.synchronize()
+    bat_s = time.perf_counter() - t0
+    same = all(np.array_equal(a.series, b.series) for a, b in zip(seq, recs))
+    value = node_steps / bat_s / 1e6
+    # algorithmic bytes of one whole-batch step (all runs going) over the tuner's HIP-event time
+    alg = sum(W.bytes_per_node_step(r.sim, None, core=True) * int(np.prod(r.sim.dims())) for r in runs)
+    achieved = alg / (tune.step_ms * 1e-3) / 1e9 if tune.step_ms > 0 else None
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": int(max(r.extras.max_steps for r in runs)), "warmup": 0,
+        "ms_per_step": round(bat_s * 1e3 / max(r.extras.max_steps for r in runs), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (96 seeded runs)",
+        "config": {"workload": "batched sweep", "runs": len(runs), "dims": "U[48,200] per axis", "pml": 8,
+                   "ports": "1-4, one driven", "steps": "U[800,3000]", "spectra_freqs": 64,
+                   "node_updates": node_steps, "build_s": round(build_s, 2), "gpu": gpu.name},
+        "sweep": {"batched_s": round(bat_s, 3), "sequential_s": round(seq_s, 3),
+                  "speedup": round(seq_s / bat_s, 3),
+                  "sequential_mcells": round(node_steps / seq_s / 1e6, 2),
+                  "records_identical": same,
+                  "tune": {"tj": tune.used[0], "seg": tune.used[1], "chunk": tune.used[2],
+                           "tune_ms": round(tune.tune_ms, 1), "from_cache": tune.from_cache}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "alg_bytes_per_launch": round(alg),
+                     "kernel": "k_march_batch (+ k_fix_batch, k_observe_batch): one step of all 96 runs",
+                     "kernel_ms": round(tune.step_ms, 4)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline("sweep")
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain"])
+    ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain", "sweep"])
     ap.add_argument("--engine", default="fused", choices=["fused", "split"])
     ap.add_argument("--timed-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -102,6 +172,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU halo exchange is not "
               "part of this build", file=sys.stderr)
         return 2
+    if args.workload == "sweep":
+        return sweep(args)
     from fieldgrid_amd import abi
     from fieldgrid_amd import fdtd as F
     from fieldgrid_amd import workloads as W

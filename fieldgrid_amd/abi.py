@@ -1,4 +1,4 @@
-"""ctypes declarations of include/fieldgrid.h and include/fieldgrid_sim.h, and the loader.
+"""ctypes declarations of include/*.h (engine, builder, batched sweep), and the loader.
 
 This is the binding a Python caller (or a test playing the reference's Rust caller) uses to
 reach libfieldgrid.so. Loading fails loudly when the library has not been built: there is
@@ -121,9 +121,19 @@ class fg_sim_desc(C.Structure):
     ]
 
 
+class fg_tune(C.Structure):
+    _fields_ = [("tj", u32), ("seg", u32), ("chunk", u32), ("pad", u32), ("cache", C.c_char_p),
+                ("used_tj", u32), ("used_seg", u32), ("used_chunk", u32), ("from_cache", u32),
+                ("tune_ms", f64), ("step_ms", f64)]
+
+
+class fg_spectra(C.Structure):
+    _fields_ = [("nf", u32), ("pad", u32), ("freqs", P(f64)), ("v", P(f64)), ("i", P(f64))]
+
+
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, u64, f64)
 
-# every symbol the two headers declare, with its ctypes signature
+# every symbol the headers declare, with its ctypes signature
 SIGNATURES = {
     "fg_open": (C.c_int, [C.c_int, P(C.c_void_p)]),
     "fg_close": (None, [C.c_void_p]),
@@ -151,6 +161,8 @@ SIGNATURES = {
     "fg_sim_counts": (None, [C.c_void_p, P(u64)]),
     "fg_sim_pack": (C.c_int, [C.c_void_p, u64, P(fg_problem)]),
     "fg_patches": (u64, [C.c_void_p, u32, P(u32), P(f64), P(f64)]),
+    "fg_run_batch": (C.c_int, [C.c_void_p, u32, P(P(fg_problem)), P(u32), P(fg_pulse), P(P(fg_extras)),
+                               P(fg_tune), PROGRESS_FN, C.c_void_p, P(fg_record), P(fg_spectra)]),
 }
 
 _lib = None

@@ -14,8 +14,11 @@ LIB = os.path.join(HERE, "libfieldgrid.so")
 ARCH = os.environ.get("FIELDGRID_ARCH", "gfx950")
 
 # Packed-FP32 instructions are switched off for all device code (-packed-fp32-ops).
+# No FMA contraction (-ffp-contract=off): every update rounds after each multiply and add in the
+# reference's operation order, so results do not depend on how the compiler fused each inlined
+# copy of a kernel body, and match the reference's CPU path operation for operation.
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
@@ -35,10 +38,12 @@ def build(verbose=True):
     os.makedirs(tmp, exist_ok=True)
     eng = os.path.join(tmp, "engine.o")
     sim = os.path.join(tmp, "sim.o")
+    bat = os.path.join(tmp, "batch.o")
     _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng])
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", bat])
     _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sim.cpp"), "-o", sim])
     out_tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, sim, "-o", out_tmp, "-lpthread"])
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, bat, sim, "-o", out_tmp, "-lpthread"])
     os.replace(out_tmp, LIB)
     return LIB
 

@@ -1,0 +1,499 @@
+// Batched sweep behind include/fieldgrid_batch.h: many independent FDTD runs stepped together.
+//
+// The reference runs one problem per run() (crates/sims-field/src/fdtd/run.rs:390-758). A sweep
+// of small problems that way leaves an MI355X mostly idle: a 100^3 grid is ~50 march blocks on
+// 256 CUs, and every step pays five launch latencies. Here every run keeps its own session
+// (fields, psi, coefficient tables, ports, records: engine.hip), and one table of per-run
+// descriptors (BatchRun) lets a single launch serve blocks of any run:
+//   * the march blocks of all runs still going are merged into lists per kernel variant
+//     (CPML class x coefficient-index width x LDS-staged sets), ordered for the XCDs;
+//   * the sparse fixes and observers of all runs are one launch each (blockIdx.y = run);
+//   * all runs start at step 0 and advance in lockstep, so they share one step counter and
+//     one ping-pong parity; a run that stops leaves the lists;
+//   * chunks of steps are captured in a hipGraph and replayed;
+//   * rows per block, planes per block and the chunk length are autotuned per batch shape and
+//     cached in a JSON file.
+// Each run's arithmetic is the single-run engine's, node for node, so a batched run's record
+// is bit-identical to an fg_run of the same problem.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fieldgrid_batch.h"
+#include "host.h"
+
+namespace fg {
+
+// Everything a kernel needs to step one run, reachable through one table index.
+struct BatchRun {
+    Geo g;
+    float *E[2], *H[2], *psi[2];
+    FixArgs fix;
+    ObsArgs obs;
+};
+
+struct BatchBlock {
+    uint32_t bx, by, bz, run;
+};
+
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ + 2)) void k_march_batch(const BatchRun *__restrict__ runs,
+                                                               const BatchBlock *__restrict__ blocks, uint32_t par,
+                                                               uint32_t seg) {
+    const BatchBlock bb = blocks[blockIdx.x];
+    if (bb.bx == ~0u) return;  // padding of the XCD-ordered list (uniform over the block)
+    const BatchRun &r = runs[bb.run];
+    march<TJ, PML, WIDE, LDS_SETS>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
+                                   r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg);
+}
+
+__global__ __launch_bounds__(256) void k_fix_batch(const BatchRun *__restrict__ runs, const uint32_t *__restrict__ list,
+                                                   uint32_t par, uint32_t s_off) {
+    const BatchRun &r = runs[list[blockIdx.y]];
+    fix_step(r.g, r.fix, r.E[par ^ 1u], r.E[par], s_off, uint64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_observe_batch(const BatchRun *__restrict__ runs,
+                                                       const uint32_t *__restrict__ list, uint32_t par,
+                                                       uint32_t s_off) {
+    const BatchRun &r = runs[list[blockIdx.y]];
+    observe(r.g, r.obs, r.E[par ^ 1u], r.H[par ^ 1u], s_off, blockIdx.x);
+}
+
+__global__ void k_batch_set_step(uint32_t *step, uint32_t value) { step[0] = value; }
+__global__ void k_batch_add_step(uint32_t *step, uint32_t value) { step[0] += value; }
+
+// One (run, port, frequency) of the port spectra: the f64 phasor recursion of
+// crates/sims-antenna/src/fdtd/mod.rs:231-246 over that port's (V, I) pairs.
+struct SpecJob {
+    const float *series;
+    uint64_t steps;
+    uint32_t ports, port;
+    double dt, w;
+};
+
+__global__ __launch_bounds__(256) void k_spectra(const SpecJob *__restrict__ jobs, uint32_t n, double *__restrict__ out) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const SpecJob j = jobs[t];
+    const double re = cos(-j.w * j.dt), ri = sin(-j.w * j.dt);   // rot_e = e^{-i w dt}
+    double pr = j.dt * re, pi = j.dt * ri;                        // pe = dt e^{-i w dt}
+    double qr = j.dt * cos(-j.w * 0.5 * j.dt), qi = j.dt * sin(-j.w * 0.5 * j.dt);  // ph
+    double vr = 0, vi = 0, ir = 0, ii = 0;
+    for (uint64_t s = 0; s < j.steps; s++) {
+        const double v = j.series[(s * j.ports + j.port) * 2], c = j.series[(s * j.ports + j.port) * 2 + 1];
+        vr += pr * v, vi += pi * v;
+        ir += qr * c, ii += qi * c;
+        const double pr2 = pr * re - pi * ri, pi2 = pr * ri + pi * re;
+        const double qr2 = qr * re - qi * ri, qi2 = qr * ri + qi * re;
+        pr = pr2, pi = pi2, qr = qr2, qi = qi2;
+    }
+    out[4 * t] = vr, out[4 * t + 1] = vi, out[4 * t + 2] = ir, out[4 * t + 3] = ii;
+}
+
+// ---- host -----------------------------------------------------------------------------------
+
+constexpr int N_LISTS = 12;  // CPML class (3) x wide index (2) x LDS sets (2)
+
+struct Batch {
+    fg_device *dev = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<Session>> runs;
+    DevBuf<BatchRun> table;
+    DevBuf<uint32_t> step;
+    DevBuf<BatchBlock> lists[N_LISTS];
+    uint32_t list_n[N_LISTS] = {};
+    DevBuf<uint32_t> fix_list, obs_list;
+    uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0;
+    uint32_t tj = 14, seg = 32, chunk = 32, xcds = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Batch() {
+        if (stream) (void)hipStreamSynchronize(stream);  // an error may leave work in flight
+        drop_graph();
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Rebuild the merged block lists and the fix/observe run lists for the runs still going.
+static void plan(Batch &b, const std::vector<uint8_t> &active) {
+    b.drop_graph();
+    std::vector<BatchBlock> L[N_LISTS];
+    std::vector<MarchBlock> cls[3];
+    std::vector<uint32_t> fl, ol;
+    b.fix_gx = b.obs_gx = 0;
+    for (uint32_t r = 0; r < b.runs.size(); r++) {
+        if (!active[r]) continue;
+        const Session &s = *b.runs[r];
+        march_classes(s, b.tj, b.seg, cls);
+        const int key = (s.geo.wide ? 2 : 0) + (s.geo.n_sets <= uint32_t(SETS_LDS) ? 1 : 0);
+        for (int c = 0; c < 3; c++)
+            for (const MarchBlock &m : cls[c]) L[c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
+        if (s.fix_blocks) fl.push_back(r), b.fix_gx = std::max(b.fix_gx, s.fix_blocks);
+        if (s.obs_blocks) ol.push_back(r), b.obs_gx = std::max(b.obs_gx, s.obs_blocks);
+    }
+    for (int q = 0; q < N_LISTS; q++) {
+        xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
+        b.lists[q].upload(L[q].data(), L[q].size(), b.stream);
+        b.list_n[q] = uint32_t(L[q].size());
+    }
+    b.fix_list.upload(fl.data(), fl.size(), b.stream);
+    b.obs_list.upload(ol.data(), ol.size(), b.stream);
+    b.n_fix = uint32_t(fl.size());
+    b.n_obs = uint32_t(ol.size());
+    FG_CHECK(hipStreamSynchronize(b.stream));
+}
+
+template <int TJ, int PML, bool W, bool L>
+static void launch_list(Batch &b, int q, uint32_t par) {
+    hipLaunchKernelGGL((k_march_batch<TJ, PML, W, L>), dim3(b.list_n[q]), dim3(64, TJ + 2), 0, b.stream, b.table.p,
+                       b.lists[q].p, par, b.seg);
+}
+
+template <int TJ>
+static void launch_march(Batch &b, uint32_t par) {
+    for (int q = 0; q < N_LISTS; q++) {
+        if (!b.list_n[q]) continue;
+        switch (q) {
+#define FG_CASE(Q, PML, W, L) \
+    case Q: launch_list<TJ, PML, W, L>(b, q, par); break;
+            FG_CASE(0, 0, false, false) FG_CASE(1, 0, false, true) FG_CASE(2, 0, true, false) FG_CASE(3, 0, true, true)
+            FG_CASE(4, 1, false, false) FG_CASE(5, 1, false, true) FG_CASE(6, 1, true, false) FG_CASE(7, 1, true, true)
+            FG_CASE(8, 2, false, false) FG_CASE(9, 2, false, true) FG_CASE(10, 2, true, false) FG_CASE(11, 2, true, true)
+#undef FG_CASE
+        }
+    }
+}
+
+// One step of every run still going, at offset s_off from the device step counter.
+static void launch_step(Batch &b, uint32_t s_off, uint32_t par) {
+    if (b.tj == 8) launch_march<8>(b, par);
+    else launch_march<14>(b, par);
+    if (b.n_fix)
+        hipLaunchKernelGGL(k_fix_batch, dim3(b.fix_gx, b.n_fix), dim3(256), 0, b.stream, b.table.p, b.fix_list.p, par,
+                           s_off);
+    if (b.n_obs)
+        hipLaunchKernelGGL(k_observe_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p, b.obs_list.p,
+                           par, s_off);
+}
+
+// Capture `chunk` steps starting at an even step, ending with the counter advanced by chunk.
+static void capture(Batch &b) {
+    b.drop_graph();
+    FG_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeRelaxed));
+    for (uint32_t q = 0; q < b.chunk; q++) launch_step(b, q, q & 1u);
+    hipLaunchKernelGGL(k_batch_add_step, dim3(1), dim3(1), 0, b.stream, b.step.p, b.chunk);
+    FG_CHECK(hipStreamEndCapture(b.stream, &b.graph));
+    FG_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+// Enqueue `count` steps from step `from` (graph replays where a whole even-aligned chunk fits).
+static void advance(Batch &b, uint64_t from, uint64_t count, bool graphs) {
+    uint64_t done = 0;
+    while (done < count) {
+        const uint64_t at = from + done;
+        hipLaunchKernelGGL(k_batch_set_step, dim3(1), dim3(1), 0, b.stream, b.step.p, uint32_t(at));
+        if (graphs && b.chunk >= 2 && (at & 1u) == 0 && count - done >= b.chunk) {
+            if (!b.exec) capture(b);
+            while (count - done >= b.chunk) {
+                FG_CHECK(hipGraphLaunch(b.exec, b.stream));
+                done += b.chunk;
+            }
+            continue;
+        }
+        // direct launches: one step to reach an even step where replays can start, or the
+        // remainder shorter than a chunk
+        const uint64_t k = graphs && b.chunk >= 2 && (at & 1u) ? 1 : count - done;
+        for (uint64_t q = 0; q < k; q++) launch_step(b, uint32_t(q), uint32_t((at + q) & 1u));
+        done += k;
+    }
+    FG_CHECK(hipGetLastError());
+}
+
+static hipEvent_t event(Batch &b, size_t i) {
+    while (b.ev.size() <= i) {
+        hipEvent_t e;
+        FG_CHECK(hipEventCreate(&e));
+        b.ev.push_back(e);
+    }
+    return b.ev[i];
+}
+
+// Mean ms per step over `steps` steps after `warm` untimed ones (graphs as in a real run).
+static double time_steps(Batch &b, uint32_t warm, uint32_t steps, bool graphs) {
+    advance(b, 0, warm, graphs);
+    FG_CHECK(hipEventRecord(event(b, 0), b.stream));
+    advance(b, warm, steps, graphs);
+    FG_CHECK(hipEventRecord(event(b, 1), b.stream));
+    FG_CHECK(hipEventSynchronize(event(b, 1)));
+    float ms = 0.f;
+    FG_CHECK(hipEventElapsedTime(&ms, event(b, 0), event(b, 1)));
+    return double(ms) / steps;
+}
+
+// ---- tuning cache: {"<shape key>": [tj, seg, chunk, step_ms], ...} ---------------------------
+
+static std::map<std::string, std::vector<double>> read_cache(const char *path) {
+    std::map<std::string, std::vector<double>> m;
+    if (!path) return m;
+    std::ifstream f(path);
+    if (!f) return m;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string t = ss.str();
+    size_t p = 0;
+    while ((p = t.find('"', p)) != std::string::npos) {
+        const size_t e = t.find('"', p + 1);
+        if (e == std::string::npos) break;
+        const std::string key = t.substr(p + 1, e - p - 1);
+        const size_t lb = t.find('[', e), rb = t.find(']', e);
+        if (lb == std::string::npos || rb == std::string::npos || rb < lb) break;
+        std::vector<double> v;
+        std::stringstream nums(t.substr(lb + 1, rb - lb - 1));
+        std::string tok;
+        while (std::getline(nums, tok, ',')) v.push_back(std::atof(tok.c_str()));
+        if (v.size() >= 3) m[key] = v;
+        p = rb + 1;
+    }
+    return m;
+}
+
+static void write_cache(const char *path, const std::map<std::string, std::vector<double>> &m) {
+    if (!path) return;
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp);
+        if (!f) return;  // an unwritable cache only loses the memo
+        f << "{\n";
+        size_t q = 0;
+        for (const auto &kv : m) {
+            f << "  \"" << kv.first << "\": [";
+            for (size_t i = 0; i < kv.second.size(); i++) f << (i ? ", " : "") << kv.second[i];
+            f << "]" << (++q < m.size() ? "," : "") << "\n";
+        }
+        f << "}\n";
+    }
+    std::rename(tmp.c_str(), path);
+}
+
+static std::string shape_key(const Batch &b) {
+    uint64_t nodes = 0, psi = 0;
+    uint32_t nmax = 0;
+    for (const auto &s : b.runs) {
+        nodes += s->nn;
+        psi += s->psi[0].n;
+        nmax = std::max({nmax, s->n[0], s->n[1], s->n[2]});
+    }
+    std::ostringstream k;
+    k << b.dev->name << "|runs=" << b.runs.size() << "|nodes=" << nodes << "|psi=" << psi << "|nmax=" << nmax;
+    return k.str();
+}
+
+static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::string key = shape_key(b);
+    auto cache = read_cache(t ? t->cache : nullptr);
+    const bool ask_tj = !t || !t->tj, ask_seg = !t || !t->seg, ask_chunk = !t || !t->chunk;
+    b.tj = t && t->tj ? (t->tj == 8 ? 8 : 14) : 14;
+    b.seg = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
+    b.chunk = t && t->chunk ? t->chunk : 32;
+    bool from_cache = false, tuned = false;
+    double step_ms = 0.0;
+    auto it = cache.find(key);
+    if ((ask_tj || ask_seg || ask_chunk) && it != cache.end()) {
+        if (ask_tj) b.tj = uint32_t(it->second[0]) == 8 ? 8 : 14;
+        if (ask_seg) b.seg = std::min<uint32_t>(std::max<uint32_t>(uint32_t(it->second[1]), 1), SEG_MAX);
+        if (ask_chunk) b.chunk = std::max<uint32_t>(uint32_t(it->second[2]), 1);
+        step_ms = it->second.size() > 3 ? it->second[3] : 0.0;
+        from_cache = true;
+    } else if (ask_tj || ask_seg || ask_chunk) {
+        tuned = true;
+        // tile shape: direct launches, so only the kernels' own time differs
+        double best = 1e30;
+        uint32_t btj = b.tj, bseg = b.seg;
+        for (uint32_t tj : {14u, 8u}) {
+            if (!ask_tj && tj != b.tj) continue;
+            for (uint32_t seg : {32u, 16u, 8u}) {
+                if (!ask_seg && seg != b.seg) continue;
+                b.tj = tj, b.seg = seg;
+                plan(b, active);
+                const double ms = time_steps(b, 2, 6, false);
+                if (ms < best) best = ms, btj = tj, bseg = seg;
+            }
+        }
+        b.tj = btj, b.seg = bseg;
+        plan(b, active);
+        step_ms = best;
+        if (ask_chunk) {
+            // graph chunk: longer chunks amortise the replay; the batch shrinks as runs end,
+            // so prefer the shortest chunk within 2 % of the best
+            double cbest = 1e30;
+            std::vector<std::pair<uint32_t, double>> got;
+            for (uint32_t c : {8u, 32u, 96u}) {
+                b.chunk = c;
+                b.drop_graph();
+                const double ms = time_steps(b, c, c, true);
+                got.push_back({c, ms});
+                cbest = std::min(cbest, ms);
+            }
+            for (auto &g : got)
+                if (g.second <= cbest * 1.02) {
+                    b.chunk = g.first;
+                    step_ms = std::min(step_ms, g.second);
+                    break;
+                }
+            b.drop_graph();
+        }
+        FG_CHECK(hipStreamSynchronize(b.stream));
+        for (auto &s : b.runs) reset(*s);
+        cache[key] = {double(b.tj), double(b.seg), double(b.chunk), step_ms};
+        write_cache(t ? t->cache : nullptr, cache);
+    }
+    if (b.chunk > 1) b.chunk += b.chunk & 1u;  // replays start on even steps
+    plan(b, active);
+    if (t) {
+        t->used_tj = b.tj, t->used_seg = b.seg, t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0;
+        t->step_ms = step_ms;
+        t->tune_ms = tuned ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
+    }
+}
+
+static void spectra(Batch &b, uint32_t count, fg_spectra *sp) {
+    std::vector<SpecJob> jobs;
+    struct Dest {
+        double *v, *i;
+        size_t at;
+    };
+    std::vector<Dest> dest;
+    for (uint32_t r = 0; r < count; r++) {
+        if (!sp || !sp[r].nf) continue;
+        const Session &s = *b.runs[r];
+        if (!sp[r].freqs || !sp[r].v || !sp[r].i) throw HipError{"fg_run_batch: spectra buffers are NULL", FG_INVALID};
+        const uint64_t steps = std::min(s.steps, s.max_steps);
+        for (uint32_t p = 0; p < s.n_ports; p++)
+            for (uint32_t f = 0; f < sp[r].nf; f++) {
+                jobs.push_back(SpecJob{s.series.p, steps, s.n_ports, p, s.dt, 2.0 * 3.14159265358979323846 * sp[r].freqs[f]});
+                dest.push_back(Dest{sp[r].v, sp[r].i, (size_t(p) * sp[r].nf + f) * 2});
+            }
+    }
+    if (jobs.empty()) return;
+    DevBuf<SpecJob> dj;
+    DevBuf<double> out;
+    dj.upload(jobs.data(), jobs.size(), b.stream);
+    out.alloc(4 * jobs.size());
+    const uint32_t n = uint32_t(jobs.size());
+    hipLaunchKernelGGL(k_spectra, dim3((n + 255) / 256), dim3(256), 0, b.stream, dj.p, n, out.p);
+    FG_CHECK(hipGetLastError());
+    std::vector<double> h(4 * jobs.size());
+    FG_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, b.stream));
+    FG_CHECK(hipStreamSynchronize(b.stream));
+    for (size_t q = 0; q < jobs.size(); q++) {
+        dest[q].v[dest[q].at] = h[4 * q], dest[q].v[dest[q].at + 1] = h[4 * q + 1];
+        dest[q].i[dest[q].at] = h[4 * q + 2], dest[q].i[dest[q].at + 1] = h[4 * q + 3];
+    }
+}
+
+}  // namespace fg
+
+extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
+                            const fg_pulse *pulses, const fg_extras *const *extras, fg_tune *tune,
+                            fg_progress_fn progress, void *user, fg_record *records, fg_spectra *sp) {
+    using namespace fg;
+    return guarded([&] {
+        if (!dev || (count && (!problems || !driven || !pulses || !extras || !records)))
+            throw HipError{"fg_run_batch: NULL argument", FG_INVALID};
+        Batch b;
+        b.dev = dev;
+        FG_CHECK(hipSetDevice(dev->ordinal));
+        FG_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        b.xcds = dev->cus >= 64 ? uint32_t(dev->cus / 32) : 1u;  // gfx950: 32 CUs per XCD
+        std::vector<BatchRun> table(count);
+        b.step.alloc(1);
+        for (uint32_t r = 0; r < count; r++) {
+            if (!problems[r] || !extras[r]) throw HipError{"fg_run_batch: NULL problem or extras", FG_INVALID};
+            fg_extras x = *extras[r];
+            x.engine = FG_ENGINE_FUSED;
+            b.runs.emplace_back(session_new(dev, problems[r], driven[r], &pulses[r], &x));
+            Session &s = *b.runs.back();
+            BatchRun &t = table[r];
+            t.g = s.geo;
+            for (int c = 0; c < 2; c++) t.E[c] = s.E[c].p, t.H[c] = s.H[c].p, t.psi[c] = s.psi[c].p;
+            t.fix = s.fix;
+            t.obs = s.obs;
+            t.fix.step = t.obs.step = b.step.p;  // lockstep: one counter for the whole batch
+        }
+        b.table.upload(table.data(), table.size(), b.stream);
+        std::vector<uint8_t> active(count, 0);
+        for (uint32_t r = 0; r < count; r++) active[r] = extras[r]->max_steps > 0;
+        if (count) autotune(b, tune, active);
+        // per-run decay checks at multiples of check_every and at max_steps (run.rs:681-748)
+        std::vector<uint64_t> every(count), pulse_end(count);
+        for (uint32_t r = 0; r < count; r++) {
+            every[r] = extras[r]->check_every ? extras[r]->check_every : 1000;
+            const double t0 = 4.0 * std::sqrt(std::log(10.0)) / (3.14159265358979323846 * pulses[r].fc);
+            pulse_end[r] = uint64_t(2.0 * t0 / problems[r]->dt);
+        }
+        auto next_check = [&](uint32_t r, uint64_t at) {
+            return std::min((at / every[r] + 1) * every[r], extras[r]->max_steps);
+        };
+        const bool graphs = b.chunk >= 2;
+        uint64_t global = 0;
+        bool dirty = false;
+        while (std::find(active.begin(), active.end(), 1) != active.end()) {
+            uint64_t next = UINT64_MAX;
+            for (uint32_t r = 0; r < count; r++)
+                if (active[r]) next = std::min(next, next_check(r, global));
+            if (dirty) plan(b, active), dirty = false;
+            advance(b, global, next - global, graphs);
+            global = next;
+            FG_CHECK(hipStreamSynchronize(b.stream));
+            double least = INFINITY;
+            for (uint32_t r = 0; r < count; r++) {
+                if (!active[r]) continue;
+                Session &s = *b.runs[r];
+                s.steps = global;
+                s.cur = int(global & 1u);
+                if (global % every[r] && global != extras[r]->max_steps) continue;
+                const double e = energy(s);
+                if (!std::isfinite(e))  // run.rs:736-738
+                    throw HipError{"run " + std::to_string(r) + ": the FDTD run went unstable", FG_UNSTABLE};
+                s.peak = std::max(s.peak, e);
+                s.decay = e > 0.0 && s.peak > 0.0 ? 10.0 * std::log10(s.peak / e) : 0.0;
+                least = std::min(least, s.decay);
+                const bool done = global >= extras[r]->max_steps ||
+                                  (global > std::max(pulse_end[r], extras[r]->min_steps) && s.decay >= extras[r]->decay_db);
+                if (done) {
+                    read_record(s, &records[r]);
+                    active[r] = 0;
+                    dirty = true;
+                }
+            }
+            if (progress && std::isfinite(least) && !progress(user, global, least))  // run.rs:742-744
+                throw HipError{"cancelled", FG_CANCELLED};
+        }
+        for (uint32_t r = 0; r < count; r++)
+            if (extras[r]->max_steps == 0) read_record(*b.runs[r], &records[r]);
+        spectra(b, count, sp);
+        return int(FG_OK);
+    });
+}

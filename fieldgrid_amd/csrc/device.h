@@ -1,0 +1,665 @@
+// Device-side views and kernel bodies shared by the single-run engine (engine.hip) and the
+// batched sweep (batch.hip). Every body here is a __device__ function; the __global__
+// wrappers that bind it to one run or to a table of runs live in the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fieldgrid.h"
+
+namespace fg {
+
+// ---- device-side views --------------------------------------------------------------------
+
+// One axis of the CPML/spacing tables (sim.rs:12-27), structure of arrays.
+struct AxDev {
+    const float *inv_d, *inv_dd, *raw_d, *raw_dd, *be, *ce, *bh, *ch;
+    const int *slot_e, *slot_h;
+};
+
+struct Geo {
+    uint32_t n0, n1, n2, pml;
+    uint64_t nn;
+    float k_mu;
+    uint32_t wide;
+    AxDev ax[3];
+    uint64_t psi_off[18];  // e*9 + c*3 + a  (run.rs:214-225)
+    const uint32_t *coef_index;
+    const float2 *coef_sets;
+    uint32_t n_sets;
+};
+
+template <int A>
+__device__ __forceinline__ uint32_t coord(uint32_t i, uint32_t j, uint32_t k) {
+    return A == 0 ? i : (A == 1 ? j : k);
+}
+
+template <int A>
+__device__ __forceinline__ uint32_t dim(const Geo &g) {
+    return A == 0 ? g.n0 : (A == 1 ? g.n1 : g.n2);
+}
+
+template <int A>
+__device__ __forceinline__ uint64_t stride(const Geo &g) {
+    return A == 0 ? uint64_t(g.n1) * g.n2 : (A == 1 ? uint64_t(g.n2) : uint64_t(1));
+}
+
+// psi_idx (sim.rs:659-665) plus the slab's base offset
+template <int A>
+__device__ __forceinline__ uint64_t psi_at(const Geo &g, int e, int c, uint32_t i, uint32_t j, uint32_t k, uint32_t s) {
+    const uint64_t b = g.psi_off[e * 9 + c * 3 + A];
+    const uint32_t w = 2u * g.pml;
+    if (A == 0) return b + (uint64_t(s) * g.n1 + j) * g.n2 + k;
+    if (A == 1) return b + (uint64_t(i) * w + s) * g.n2 + k;
+    return b + (uint64_t(i) * g.n1 + j) * w + s;
+}
+
+__device__ __forceinline__ float2 coef_at(const Geo &g, uint64_t f) {
+    const uint32_t s = g.wide ? g.coef_index[f] : uint32_t(reinterpret_cast<const uint16_t *>(g.coef_index)[f]);
+    return g.coef_sets[s];
+}
+
+// New H component C at node (i,j,k): H -= k_mu*(dEv/du - dEu/dv + psi terms) (cpu.rs:48-107).
+// Reads E/H/psi from the "old" buffers; writes psi only when psi_new is non-null (the
+// owner of the node), so tiles may recompute a neighbour's H for their halo.
+template <int C>
+__device__ __forceinline__ float h_next(const Geo &g, const float *__restrict__ E, const float *__restrict__ H,
+                                        const float *psi_old, float *psi_new, uint32_t i, uint32_t j, uint32_t k) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const float hold = H[C * g.nn + id];
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    if (pu + 1 >= dim<U>(g) || pv + 1 >= dim<V>(g)) return hold;
+    const float ev = E[V * g.nn + id + stride<U>(g)] - E[V * g.nn + id];
+    const float eu = E[U * g.nn + id + stride<V>(g)] - E[U * g.nn + id];
+    float t1 = ev * g.ax[U].inv_d[pu];
+    float t2 = eu * g.ax[V].inv_d[pv];
+    if (g.pml) {
+        const int su = g.ax[U].slot_h[pu];
+        if (su >= 0) {
+            const uint64_t q = psi_at<U>(g, 1, C, i, j, k, uint32_t(su));
+            const float r = g.ax[U].bh[pu] * psi_old[q] + g.ax[U].ch[pu] * ev * g.ax[U].raw_d[pu];
+            if (psi_new) psi_new[q] = r;
+            t1 += r;
+        }
+        const int sv = g.ax[V].slot_h[pv];
+        if (sv >= 0) {
+            const uint64_t q = psi_at<V>(g, 1, C, i, j, k, uint32_t(sv));
+            const float r = g.ax[V].bh[pv] * psi_old[q] + g.ax[V].ch[pv] * eu * g.ax[V].raw_d[pv];
+            if (psi_new) psi_new[q] = r;
+            t2 += r;
+        }
+    }
+    return hold - g.k_mu * (t1 - t2);
+}
+
+// New E component C from the H differences hv = H_v(p)-H_v(p-u), hu = H_u(p)-H_u(p-v)
+// (cpu.rs:109-167). Edges with cb == 0 (PEC, outer faces) keep their value.
+template <int C>
+__device__ __forceinline__ float e_next(const Geo &g, float eold, float hv, float hu, uint32_t i, uint32_t j, uint32_t k,
+                                        uint64_t id, const float *psi_old, float *psi_new) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const float2 cf = coef_at(g, C * g.nn + id);
+    if (cf.y == 0.f) return eold;
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    float t1 = hv * g.ax[U].inv_dd[pu];
+    float t2 = hu * g.ax[V].inv_dd[pv];
+    if (g.pml) {
+        const int su = g.ax[U].slot_e[pu];
+        if (su >= 0) {
+            const uint64_t q = psi_at<U>(g, 0, C, i, j, k, uint32_t(su));
+            const float r = g.ax[U].be[pu] * psi_old[q] + g.ax[U].ce[pu] * hv * g.ax[U].raw_dd[pu];
+            psi_new[q] = r;
+            t1 += r;
+        }
+        const int sv = g.ax[V].slot_e[pv];
+        if (sv >= 0) {
+            const uint64_t q = psi_at<V>(g, 0, C, i, j, k, uint32_t(sv));
+            const float r = g.ax[V].be[pv] * psi_old[q] + g.ax[V].ce[pv] * hu * g.ax[V].raw_dd[pv];
+            psi_new[q] = r;
+            t2 += r;
+        }
+    }
+    return cf.x * eold + cf.y * (t1 - t2);
+}
+
+// ---- fused engine: one H->E pass per step, ping-pong buffers ------------------------------
+//
+// The reference fuses H and E per step with 64x4 tiles and recomputed halos
+// (fdtd.wgsl:186-330). Here a block is a column of ROWS = TJ+2 waves marching along i (the
+// slowest axis) over `seg` planes:
+//   * wave w holds row j = j0-1+w; rows 1..TJ are owned, row 0 recomputes new Hx/Hz at
+//     j0-1 (the E update needs them), row TJ+1 only supplies E at j0+TJ (the H update needs it);
+//   * lane L holds k = k0-1+L with k0 = 62*blockIdx.x; lanes 1..62 are owned, lanes 0 and 63
+//     are k-halos whose values reach their neighbours through cross-lane shuffles;
+//   * E(i) and new Hy/Hz(i-1) ride in registers from one plane to the next, so each plane
+//     loads E(i+1) and H(i) once, and rows trade E and new H through LDS behind one barrier.
+// Halo work reads only the old buffers and writes nothing, so no block reads what another
+// block writes in the same step. Every node's own E, H, coefficient index and CPML psi are
+// read once and written once per step.
+constexpr int OWN_K = 62;
+constexpr int SETS_LDS = 2048;  // coefficient sets staged in LDS per block (16 KB)
+constexpr int SEG_MAX = 64;     // longest i segment one march block takes
+
+// Block coordinates of one march block (k tile, j tile, i segment). Blocks that touch a CPML
+// slab and blocks that do not are launched as two lists, so the interior kernel carries no
+// psi code at all.
+struct MarchBlock {
+    uint32_t bx, by, bz;
+};
+
+// Raw buffer access: a plane-uniform base pointer in a buffer resource (scalar registers)
+// and a 32-bit per-lane byte offset, so no lane carries a 64-bit address. `bytes` bounds the
+// resource: an access past it reads 0 / is dropped by the hardware instead of faulting.
+// 0x00020000 is the gfx9-family (incl. gfx950) resource word 3 for raw dword access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(bytes), 0x00020000);
+}
+__device__ __forceinline__ float ld(const float *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc(p, bytes), int(byte_off), 0, 0));
+}
+__device__ __forceinline__ uint32_t ld(const uint32_t *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rsrc(p, bytes), int(byte_off), 0, 0);
+}
+__device__ __forceinline__ uint16_t ld(const uint16_t *p, uint32_t byte_off, uint32_t bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b16(rsrc(p, bytes), int(byte_off), 0, 0);
+}
+__device__ __forceinline__ void st(float *p, uint32_t byte_off, uint32_t bytes, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc(p, bytes), int(byte_off), 0, 0);
+}
+
+constexpr uint32_t OOB = 0x7FFFFFF0u;  // a byte offset past every resource: load 0, store dropped
+
+// psi = b*old + c*diff*raw (cpu.rs:78-99, 137-157), written to the new store at `off`.
+__device__ __forceinline__ float cpml_next(float *pn, uint32_t bytes, uint32_t off, float b, float c, float raw,
+                                           float old, float diff) {
+    const float r = b * old + c * diff * raw;
+    st(pn, off, bytes, r);
+    return r;
+}
+
+// One CPML psi recursion, psi = b*psi + c*diff*raw (cpu.rs:78-99, 137-157), from the old store
+// to the new one; po/pn are plane-uniform bases of the psi array, `off` the lane's byte offset.
+__device__ __forceinline__ float cpml(const float *po, float *pn, uint32_t bytes, uint32_t off, float b, float c,
+                                      float raw, float diff, bool store) {
+    const float r = b * ld(po, off, bytes) + c * diff * raw;
+    if (store) st(pn, off, bytes, r);
+    return r;
+}
+
+
+// PML: 0 = no CPML slab in reach, 1 = only z (k) slabs, 2 = any slab (x, y and z psi code)
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__device__ __forceinline__ void march(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
+                                      float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
+                                      const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg) {
+    constexpr int ROWS = TJ + 2;
+    // row exchanges; rows are stored at index w+1 so that rows -1 and ROWS are zero pads
+    __shared__ float xe[2][ROWS + 2][2][64];  // E(i+1) Ex, Ez of each row, for the row above
+    __shared__ float xh[2][ROWS + 2][2][64];  // new H(i) Hx, Hz of each row, for the row below
+    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    __shared__ float xa[SEG_MAX + 1][PXY ? 10 : 2];  // x-axis values of the segment's planes
+    // blockDim.x is one wave, so the row index is wave-uniform: keep it (and every per-row
+    // value) in scalar registers
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k = int(mb.bx) * OWN_K + lane - 1;
+    const int j = int(mb.by) * TJ + w - 1;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 1 && lane <= OWN_K && w >= 1 && w <= TJ;
+    const bool bottom = w == TJ + 1;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;  // k+1 / j+1 inside the grid
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    // byte offsets of this lane; a lane that must not touch a plane gets OOB instead, so
+    // every access below is unconditional (no exec-masked branches around memory ops,
+    // which would make the compiler drain all outstanding loads)
+    const uint32_t ob4 = off * 4u;
+    const uint32_t ldE = act ? ob4 : OOB, ldH = act && !bottom ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = own ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    // CPML slots and recursion coefficients of this lane (k) and row (j), and the lane's
+    // loop-invariant byte offsets into the z- and y-slab psi planes
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
+    if constexpr (PML) {
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
+    }
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    // per-plane (x) axis values staged in LDS: plain loads of them inside the loop would queue
+    // behind the prefetches and force the wave to wait for all of them
+    const uint32_t istart = i0 > 0 ? i0 - 1 : 0;
+    for (uint32_t t = w * 64 + lane; t < i1 - istart; t += 64 * ROWS) {
+        const uint32_t q = istart + t;
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        if constexpr (PXY) {
+            xa[t][2] = g.ax[0].bh[q], xa[t][3] = g.ax[0].ch[q], xa[t][4] = g.ax[0].raw_d[q];
+            xa[t][5] = g.ax[0].be[q], xa[t][6] = g.ax[0].ce[q], xa[t][7] = g.ax[0].raw_dd[q];
+            xa[t][8] = __int_as_float(g.ax[0].slot_h[q]), xa[t][9] = __int_as_float(g.ax[0].slot_e[q]);
+        }
+    }
+    // zero pads (rows -1 and ROWS of both buffers)
+    if (w < 2) {
+        for (int q = 0; q < 2; q++) {
+            xe[q][w * (ROWS + 1)][0][lane] = 0.f, xe[q][w * (ROWS + 1)][1][lane] = 0.f;
+            xh[q][w * (ROWS + 1)][0][lane] = 0.f, xh[q][w * (ROWS + 1)][1][lane] = 0.f;
+        }
+    }
+    // software pipeline: E(i+1), H(i) and the coefficient indices of plane i are loaded one
+    // plane ahead into the other of two register sets; the loop is unrolled by two so the sets
+    // swap roles without copies (a copy would have to wait for the loads in flight)
+    struct Pipe {
+        float ex1, ey1, ez1, hx, hy, hz;
+        uint32_t c0, c1, c2;
+    };
+    Pipe A, B;
+    {
+        const float *p = Eo + uint64_t(istart) * plane;
+        B.ex1 = ld(p, ldE, pl4), B.ey1 = ld(p + nn, ldE, pl4), B.ez1 = ld(p + 2 * nn, ldE, pl4);
+    }
+    // E(istart) Ex/Ez of the row below, through the buffer the first plane does not write
+    const int pb0 = (istart + 1) & 1;
+    xe[pb0][w + 1][0][lane] = B.ex1;
+    xe[pb0][w + 1][1][lane] = B.ez1;
+    __syncthreads();
+    float exn = xe[pb0][w + 2][0][lane], ezn = xe[pb0][w + 2][1][lane];
+    {
+        const uint64_t pb = uint64_t(istart) * plane;
+        const uint32_t eb = istart + 1 < g.n0 ? pl4 : 0u;  // a zero-size resource reads 0
+        const float *p = Eo + pb + (istart + 1 < g.n0 ? plane : 0);
+        A.ex1 = ld(p, ldE, eb), A.ey1 = ld(p + nn, ldE, eb), A.ez1 = ld(p + 2 * nn, ldE, eb);
+        const float *q = Ho + pb;
+        A.hx = ld(q, ldH, pl4), A.hy = ld(q + nn, ldH, pl4), A.hz = ld(q + 2 * nn, ldH, pl4);
+        A.c0 = cidx(pb, plc), A.c1 = cidx(nn + pb, plc), A.c2 = cidx(2 * nn + pb, plc);
+    }
+    float hy_p = 0.f, hz_p = 0.f;
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    auto plane_step = [&](uint32_t i, Pipe &cur, Pipe &nxt) {
+        const int b = i & 1;
+        const bool wplane = i >= i0;
+        const bool il = i + 1 < g.n0;
+        const uint64_t pb = uint64_t(i) * plane;  // uniform
+        // E(i) arrived two planes ago in the set this plane's prefetch now refills
+        const float ex = nxt.ex1, ey = nxt.ey1, ez = nxt.ez1;
+        // prefetch of plane i+1 (E of plane i+2); absent planes read through zero-size resources
+        const bool more = i + 1 < i1, more2 = more && i + 2 < g.n0;
+        const float *pe = Eo + pb + (more2 ? 2 * plane : 0);
+        nxt.ex1 = ld(pe, ldE, more2 ? pl4 : 0u), nxt.ey1 = ld(pe + nn, ldE, more2 ? pl4 : 0u),
+        nxt.ez1 = ld(pe + 2 * nn, ldE, more2 ? pl4 : 0u);
+        const float *ph = Ho + pb + (more ? plane : 0);
+        nxt.hx = ld(ph, ldH, more ? pl4 : 0u), nxt.hy = ld(ph + nn, ldH, more ? pl4 : 0u),
+        nxt.hz = ld(ph + 2 * nn, ldH, more ? pl4 : 0u);
+        const uint64_t cb_ = pb + (more ? plane : 0);
+        nxt.c0 = cidx(cb_, more ? plc : 0u), nxt.c1 = cidx(nn + cb_, more ? plc : 0u),
+        nxt.c2 = cidx(2 * nn + cb_, more ? plc : 0u);
+        const float *xv = xa[i - istart];
+        const float idx_ = xv[0], iddx = xv[1];
+        // CPML: every psi value this lane needs on plane i is loaded here, up front, so the
+        // loads overlap. A slot that does not exist maps to an offset past its resource (the
+        // load returns 0 and the store is dropped) and has b = c = 0, so its term adds exactly 0.
+        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
+        float hxy = 0, hxz = 0, hyz = 0, hyx = 0, hzx = 0, hzy = 0;  // old H psi (component, axis)
+        float exy = 0, exz = 0, eyz = 0, eyx = 0, ezx = 0, ezy = 0;  // old E psi
+        uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
+        uint32_t oxh = OOB, oxe = OOB;
+        if constexpr (PZ) {
+            zrow = uint64_t(i) * g.n1 * p2;  // a = 2: (i*n1 + j)*2p + s
+            const uint32_t zh = ldH == OOB ? OOB : ozh, ze = stO == OOB ? OOB : oze;
+            hxz = ld(psio + PSI(1, 0, 2) + zrow, zh, zbytes), hyz = ld(psio + PSI(1, 1, 2) + zrow, zh, zbytes);
+            exz = ld(psio + PSI(0, 0, 2) + zrow, ze, zbytes), eyz = ld(psio + PSI(0, 1, 2) + zrow, ze, zbytes);
+        }
+        if constexpr (PXY) {
+            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
+            bhx = xv[2], chx = xv[3], rdx = xv[4];
+            bex = xv[5], cex = xv[6], rddx = xv[7];
+            yrow = uint64_t(i) * p2 * g.n2;  // a = 1: (i*2p + s)*n2 + k
+            xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;  // a = 0
+            oxh = shx >= 0 ? ldE : OOB, oxe = sex >= 0 ? stO : OOB;
+            const uint32_t yh = ldH == OOB ? OOB : oyh, xh_ = ldH == OOB ? OOB : oxh;
+            hxy = ld(psio + PSI(1, 0, 1) + yrow, yh, ybytes);
+            hyx = ld(psio + PSI(1, 1, 0) + xhb, xh_, pl4);
+            hzx = ld(psio + PSI(1, 2, 0) + xhb, xh_, pl4), hzy = ld(psio + PSI(1, 2, 1) + yrow, yh, ybytes);
+            const uint32_t ye = stO == OOB ? OOB : oye;
+            exy = ld(psio + PSI(0, 0, 1) + yrow, ye, ybytes);
+            eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, pl4);
+            ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, pl4), ezy = ld(psio + PSI(0, 2, 1) + yrow, ye, ybytes);
+        }
+        // stores of this plane: only owners, only on planes of their own segment
+        const uint32_t sto = wplane ? stO : OOB;
+        // new psi from its old value; stored (where the slot exists) only by the node's owner
+#define CP(base, bytes, o, B, CC, R, old, diff) cpml_next(psin + (base), bytes, sto == OOB ? OOB : (o), B, CC, R, old, diff)
+        // k+1 neighbours of E(i) from the next lane
+        const float ex_k = __shfl_down(ex, 1, 64), ey_k = __shfl_down(ey, 1, 64);
+        float hx, hy, hz;
+        {
+            // Hx: u = y, v = z
+            const float ev = ezn - ez, eu = ey_k - ey;
+            float t1 = ev * idy, t2 = eu * idz;
+            const bool v = jl && kl;  // psi advances only where the H component does
+            if constexpr (PXY) t1 += CP(PSI(1, 0, 1) + yrow, ybytes, v ? oyh : OOB, bhy, chy, rdy, hxy, ev);
+            if constexpr (PZ) t2 += CP(PSI(1, 0, 2) + zrow, zbytes, v ? ozh : OOB, bhz, chz, rdz, hxz, eu);
+            hx = jl && kl ? cur.hx - g.k_mu * (t1 - t2) : cur.hx;
+        }
+        {
+            // Hy: u = z, v = x
+            const float ev = ex_k - ex, eu = cur.ez1 - ez;
+            float t1 = ev * idz, t2 = eu * idx_;
+            const bool v = kl && il;
+            if constexpr (PZ) t1 += CP(PSI(1, 1, 2) + zrow, zbytes, v ? ozh : OOB, bhz, chz, rdz, hyz, ev);
+            if constexpr (PXY) t2 += CP(PSI(1, 1, 0) + xhb, pl4, v ? oxh : OOB, bhx, chx, rdx, hyx, eu);
+            hy = kl && il ? cur.hy - g.k_mu * (t1 - t2) : cur.hy;
+        }
+        {
+            // Hz: u = x, v = y
+            const float ev = cur.ey1 - ey, eu = exn - ex;
+            float t1 = ev * idx_, t2 = eu * idy;
+            const bool v = il && jl;
+            if constexpr (PXY) t1 += CP(PSI(1, 2, 0) + xhb, pl4, v ? oxh : OOB, bhx, chx, rdx, hzx, ev);
+            if constexpr (PXY) t2 += CP(PSI(1, 2, 1) + yrow, ybytes, v ? oyh : OOB, bhy, chy, rdy, hzy, eu);
+            hz = il && jl ? cur.hz - g.k_mu * (t1 - t2) : cur.hz;
+        }
+        {
+            float *q = Hn + pb;
+            st(q, sto, pl4, hx), st(q + nn, sto, pl4, hy), st(q + 2 * nn, sto, pl4, hz);
+        }
+        xh[b][w + 1][0][lane] = hx;
+        xh[b][w + 1][1][lane] = hz;
+        xe[b][w + 1][0][lane] = cur.ex1;
+        xe[b][w + 1][1][lane] = cur.ez1;
+        __syncthreads();
+        const float hx_j = xh[b][w][0][lane], hz_j = xh[b][w][1][lane];
+        const float exn1 = xe[b][w + 2][0][lane], ezn1 = xe[b][w + 2][1][lane];
+        // k-1 neighbours of new H(i) from the previous lane
+        const float hx_k = __shfl_up(hx, 1, 64), hy_k = __shfl_up(hy, 1, 64);
+        {
+            float *q = En + pb;
+            // Ex: u = y, v = z
+            {
+                const float2 cf = coef(cur.c0);
+                const float hv = hz - hz_j, hu = hy - hy_k;
+                float t1 = hv * iddy, t2 = hu * iddz;
+                const bool on = cf.y != 0.f;
+                if constexpr (PXY) t1 += CP(PSI(0, 0, 1) + yrow, ybytes, on ? oye : OOB, bey, cey, rddy, exy, hv);
+                if constexpr (PZ) t2 += CP(PSI(0, 0, 2) + zrow, zbytes, on ? oze : OOB, bez, cez, rddz, exz, hu);
+                st(q, sto, pl4, cf.y != 0.f ? cf.x * ex + cf.y * (t1 - t2) : ex);
+            }
+            // Ey: u = z, v = x
+            {
+                const float2 cf = coef(cur.c1);
+                const float hv = hx - hx_k, hu = hz - hz_p;
+                float t1 = hv * iddz, t2 = hu * iddx;
+                const bool on = cf.y != 0.f;
+                if constexpr (PZ) t1 += CP(PSI(0, 1, 2) + zrow, zbytes, on ? oze : OOB, bez, cez, rddz, eyz, hv);
+                if constexpr (PXY) t2 += CP(PSI(0, 1, 0) + xeb, pl4, on ? oxe : OOB, bex, cex, rddx, eyx, hu);
+                st(q + nn, sto, pl4, cf.y != 0.f ? cf.x * ey + cf.y * (t1 - t2) : ey);
+            }
+            // Ez: u = x, v = y
+            {
+                const float2 cf = coef(cur.c2);
+                const float hv = hy - hy_p, hu = hx - hx_j;
+                float t1 = hv * iddx, t2 = hu * iddy;
+                const bool on = cf.y != 0.f;
+                if constexpr (PXY) t1 += CP(PSI(0, 2, 0) + xeb, pl4, on ? oxe : OOB, bex, cex, rddx, ezx, hv);
+                if constexpr (PXY) t2 += CP(PSI(0, 2, 1) + yrow, ybytes, on ? oye : OOB, bey, cey, rddy, ezy, hu);
+                st(q + 2 * nn, sto, pl4, cf.y != 0.f ? cf.x * ez + cf.y * (t1 - t2) : ez);
+            }
+        }
+#undef CP
+        exn = exn1, ezn = ezn1;
+        hy_p = hy, hz_p = hz;
+    };
+    // both unrolled copies are unconditional (an odd plane count is peeled first) so the
+    // compiler keeps them apart instead of merging them back into one body with copies
+    if ((i1 - istart) & 1u) {
+        plane_step(istart, A, B);
+        for (uint32_t i = istart + 1; i < i1; i += 2) {
+            plane_step(i, B, A);
+            plane_step(i + 1, A, B);
+        }
+    } else {
+        for (uint32_t i = istart; i < i1; i += 2) {
+            plane_step(i, A, B);
+            plane_step(i + 1, B, A);
+        }
+    }
+#undef PSI
+}
+
+// ---- sparse per-step fixes: Debye ADE, port sources, inductors (cpu.rs:239-303) ----------
+
+struct FixArgs {
+    uint64_t n_debye;
+    const fg_debye_edge *debye;
+    const float *dtab;  // per set: np x (ka, kb)
+    uint32_t np;
+    float *dstate;      // np x n_debye, pole-major (run.rs:492-493)
+    const float *dold;  // split engine: E saved before the E sweep; fused: NULL (read Eo)
+    uint32_t n_src;
+    const fg_source *src;
+    uint32_t n_ind;
+    const fg_inductor *ind;
+    float *ind_i;
+    float dt, t0, tau, w0;
+    const uint32_t *step;
+    // pulse((n+1/2)dt) per step, evaluated on the host in f32 with the reference's expression
+    // (cpu.rs:262-265), so the device adds exactly the source values of the CPU path
+    const float *pulse;
+    uint32_t n_pulse;
+    // sources and inductors share an edge (or a source edge repeats): apply them one after
+    // another in one thread, in the reference's order, instead of one thread per entry
+    uint32_t serial;
+};
+
+// The same pulse on the device, for steps past the host table (a session stepped beyond
+// max_steps): f32 like cpu.rs:262-265, but with the device's expf/cosf.
+__device__ __forceinline__ float pulse_at(const FixArgs &a, float t) {
+    const float x = (t - a.t0) / a.tau;
+    return expf(-x * x) * cosf(a.w0 * (t - a.t0));
+}
+
+// Thread t of one run's sparse fixes, in the order of cpu.rs:281-303: Debye ADE, then the
+// driven port's sources, then the inductors. E is the new field, Eo the previous step's.
+__device__ __forceinline__ void fix_step(const Geo &g, const FixArgs &a, float *__restrict__ E,
+                                         const float *__restrict__ Eo, uint32_t s_off, uint64_t t) {
+    if (t < a.n_debye) {
+        const fg_debye_edge d = a.debye[t];
+        const float old = a.dold ? a.dold[t] : Eo[d.f];
+        const float cb = coef_at(g, d.f).y;
+        const float *tab = a.dtab + size_t(d.set) * a.np * 2;
+        float s0 = 0.f;
+        for (uint32_t q = 0; q < a.np; q++) s0 += 0.5f * (1.0f + tab[2 * q]) * a.dstate[q * a.n_debye + t];
+        const float next = E[d.f] - cb * s0;
+        E[d.f] = next;
+        const float de = next - old;
+        for (uint32_t q = 0; q < a.np; q++) {
+            float &st = a.dstate[q * a.n_debye + t];
+            st = tab[2 * q] * st + d.dd * tab[2 * q + 1] * de;
+        }
+    }
+    auto source = [&](uint64_t q) {
+        const uint32_t n = a.step[0] + s_off;
+        const float pl = n < a.n_pulse ? a.pulse[n] : pulse_at(a, (float(n) + 0.5f) * a.dt);
+        const fg_source s = a.src[q];
+        E[s.comp * g.nn + s.id] -= s.src * pl;
+    };
+    auto inductor = [&](uint64_t q) {
+        const fg_inductor d = a.ind[q];
+        const uint64_t f = d.comp * g.nn + d.id;
+        const float x = E[f] - d.k_e * a.ind_i[q];
+        E[f] = x;
+        a.ind_i[q] += d.k_i * x;
+    };
+    if (a.serial) {
+        if (t == 0) {
+            for (uint32_t q = 0; q < a.n_src; q++) source(q);
+            for (uint32_t q = 0; q < a.n_ind; q++) inductor(q);
+        }
+        return;
+    }
+    if (t < a.n_src) source(t);
+    if (t < a.n_ind) inductor(t);
+}
+
+// ---- per-step observers: port V/I, point probes, plane DFT, NTFF (cpu.rs:309-415) --------
+
+struct ObsArgs {
+    uint32_t n_ports;
+    const fg_port *ports;
+    const fg_port_edge *edges;
+    const fg_loop_term *loops;
+    float *series;
+    uint64_t cap;
+    uint32_t n_probes;
+    const uint64_t *probe_f;  // flat index into E (field < 3) or H (+ 3*nn marker)
+    float *probes;
+    uint32_t plane_n, plane_k;
+    float *plane;
+    uint64_t n_patches;
+    const uint32_t *patch;
+    float *ntff;
+    uint32_t nf;
+    float omega[FG_MAX_FREQS];
+    float dt;
+    uint32_t blk_ports, blk_probes, blk_plane, blk_ntff;
+    const uint32_t *step;
+    // per step and frequency: cos(w te) dt, -sin(w te) dt, cos(w th) dt, -sin(w th) dt with
+    // te = (n+1)dt, th = (n+1/2)dt, evaluated on the host in f32 as cpu.rs:352-353 and 404-405 do
+    const float *phase;
+    uint32_t n_phase;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block `blk` of one run's observers (cpu.rs:309-415): port V/I waves, then point probes,
+// then plane DFT, then NTFF patches, each section a range of 256-thread blocks.
+__device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const float *__restrict__ E,
+                                        const float *__restrict__ H, uint32_t s_off, uint32_t blk) {
+    const uint32_t n = a.step[0] + s_off;
+    const uint64_t nn = g.nn;
+    if (blk < a.blk_ports) {
+        // one wave per port; f32 products summed in f64 like cpu.rs:317-328
+        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+        if (p >= a.n_ports) return;
+        const fg_port d = a.ports[p];
+        double v = 0.0, cur = 0.0;
+        for (uint32_t q = lane; q < d.n_edges; q += 64) {
+            const fg_port_edge pe = a.edges[d.first_edge + q];
+            v -= double(E[d.comp * nn + pe.id] * pe.w);
+        }
+        for (uint32_t q = lane; q < d.n_loops; q += 64) {
+            const fg_loop_term lt = a.loops[d.first_loop + q];
+            cur += double(H[lt.comp * nn + lt.id] * lt.w);
+        }
+        v = wave_sum(v);
+        cur = wave_sum(cur);
+        if (lane == 0 && n < a.cap) {
+            a.series[(uint64_t(n) * a.n_ports + p) * 2] = float(v);
+            a.series[(uint64_t(n) * a.n_ports + p) * 2 + 1] = float(cur);
+        }
+        return;
+    }
+    blk -= a.blk_ports;
+    if (blk < a.blk_probes) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q < a.n_probes && n < a.cap) {
+            const uint64_t f = a.probe_f[q];
+            a.probes[uint64_t(n) * a.n_probes + q] = f < 3 * nn ? E[f] : H[f - 3 * nn];
+        }
+        return;
+    }
+    blk -= a.blk_probes;
+    const float dt = a.dt;
+    const float te = (float(n) + 1.0f) * dt, th = (float(n) + 0.5f) * dt;
+    if (blk < a.blk_plane) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q >= a.plane_n) return;
+        const uint32_t i = q / g.n1, j = q % g.n1;
+        const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + a.plane_k;
+        const float f[5] = {E[id], E[nn + id], E[2 * nn + id], H[id], H[nn + id]};
+        float *out = a.plane + uint64_t(q) * a.nf * 10;
+        for (uint32_t kf = 0; kf < a.nf; kf++) {
+            const float w = a.omega[kf];
+            const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
+            const bool tab = n < a.n_phase;
+            const float ce0 = tab ? ph[0] : cosf(w * te) * dt, ce1 = tab ? ph[1] : -sinf(w * te) * dt;
+            const float ch0 = tab ? ph[2] : cosf(w * th) * dt, ch1 = tab ? ph[3] : -sinf(w * th) * dt;
+            for (int c = 0; c < 5; c++) {
+                const float p0 = c >= 3 ? ch0 : ce0, p1 = c >= 3 ? ch1 : ce1;
+                out[kf * 10 + 2 * c] += f[c] * p0;
+                out[kf * 10 + 2 * c + 1] += f[c] * p1;
+            }
+        }
+        return;
+    }
+    blk -= a.blk_plane;
+    if (blk < a.blk_ntff) {
+        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        if (gi >= a.n_patches) return;
+        const uint32_t *ix = a.patch + gi * 13;
+        const uint32_t meta = ix[0];
+        const uint32_t ax = meta & 3u;
+        const float side = (meta & 4u) ? 1.f : -1.f;
+        const uint32_t u = (ax + 1) % 3, v = (ax + 2) % 3;
+        float ef[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
+        ef[u] = 0.5f * (E[u * nn + ix[1]] + E[u * nn + ix[2]]);
+        ef[v] = 0.5f * (E[v * nn + ix[3]] + E[v * nn + ix[4]]);
+        hf[u] = 0.25f * (H[u * nn + ix[5]] + H[u * nn + ix[6]] + H[u * nn + ix[7]] + H[u * nn + ix[8]]);
+        hf[v] = 0.25f * (H[v * nn + ix[9]] + H[v * nn + ix[10]] + H[v * nn + ix[11]] + H[v * nn + ix[12]]);
+        float nrm[3] = {0.f, 0.f, 0.f};
+        nrm[ax] = side;
+        const float jv[3] = {nrm[1] * hf[2] - nrm[2] * hf[1], nrm[2] * hf[0] - nrm[0] * hf[2],
+                             nrm[0] * hf[1] - nrm[1] * hf[0]};
+        const float mv[3] = {ef[1] * nrm[2] - ef[2] * nrm[1], ef[2] * nrm[0] - ef[0] * nrm[2],
+                             ef[0] * nrm[1] - ef[1] * nrm[0]};
+        float *out = a.ntff + gi * a.nf * 12;
+        for (uint32_t kf = 0; kf < a.nf; kf++) {
+            const float wf = a.omega[kf];
+            // cos and sin are exactly even and odd, so the table's values are these
+            const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
+            const bool tab = n < a.n_phase;
+            const float pe0 = tab ? ph[0] : cosf(-wf * te) * dt, pe1 = tab ? ph[1] : sinf(-wf * te) * dt;
+            const float ph0 = tab ? ph[2] : cosf(-wf * th) * dt, ph1 = tab ? ph[3] : sinf(-wf * th) * dt;
+            float *o = out + kf * 12;
+            for (int c = 0; c < 3; c++) {
+                o[2 * c] += ph0 * jv[c];
+                o[2 * c + 1] += ph1 * jv[c];
+                o[6 + 2 * c] += pe0 * mv[c];
+                o[6 + 2 * c + 1] += pe1 * mv[c];
+            }
+        }
+    }
+}
+
+}  // namespace fg

@@ -1,0 +1,160 @@
+// Host-side state shared by the single-run engine (engine.hip) and the batched sweep
+// (batch.hip): errors, device buffers, the device handle and one run's session.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fieldgrid.h"
+#include "common.h"
+#include "device.h"
+
+namespace fg {
+
+extern thread_local std::string g_error;
+
+struct HipError {
+    std::string msg;
+    int status;
+};
+
+#define FG_CHECK(expr)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::fg::HipError{std::string(#expr) + ": " + hipGetErrorString(e_),                \
+                                 e_ == hipErrorOutOfMemory ? FG_TOO_LARGE : FG_HIP};               \
+    } while (0)
+
+constexpr int ENERGY_BLOCKS = 1024;
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count) FG_CHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void zero(hipStream_t s) {
+        if (n) FG_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    void upload(const T *src, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) FG_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+}  // namespace fg
+
+struct fg_device {
+    int ordinal;
+    std::string name;
+    uint64_t total_mem;
+    int cus;
+};
+
+namespace fg {
+
+struct Session {
+    fg_device *dev;
+    hipStream_t stream = nullptr;
+    Geo geo{};
+    FixArgs fix{};
+    ObsArgs obs{};
+    uint64_t nn = 0;
+    uint32_t n[3];
+    double dt = 0.0;  // Sim.dt
+    bool fused = true;
+    uint32_t tj = 14, seg = 32, xcds = 8;
+    uint64_t max_steps = 0, steps = 0;
+    uint32_t n_ports = 0, n_probes = 0, nf = 0;
+    uint64_t plane_len = 0, ntff_len = 0;
+    double peak = 0.0, decay = 0.0;
+    int cur = 0;  // which ping-pong copy holds the latest fields (fused)
+    uint32_t fix_blocks = 0, obs_blocks = 0;
+    bool timing = false;
+    double kernel_ms[3] = {0, 0, 0};  // step kernel(s), fix, observe
+    uint64_t kernel_count = 0, pending_steps = 0;
+    // buffers
+    DevBuf<float> E[2], H[2], psi[2];
+    DevBuf<uint32_t> coef_index, step;
+    DevBuf<float> coef_sets, axes, dtab, dstate, dold, ind_i, series, plane, ntff, probes, pulse, phase;
+    DevBuf<int> slots;
+    DevBuf<fg_debye_edge> debye;
+    DevBuf<fg_source> src;
+    DevBuf<fg_inductor> ind;
+    DevBuf<fg_port> ports;
+    DevBuf<fg_port_edge> edges;
+    DevBuf<fg_loop_term> loops;
+    DevBuf<uint64_t> probe_f;
+    DevBuf<uint32_t> patch;
+    DevBuf<double> partial;
+    DevBuf<MarchBlock> blk_int, blk_z, blk_pml;  // march blocks clear of CPML / touching z slabs only / any
+    std::vector<hipEvent_t> events;
+    // host copies the batched sweep and reset() need
+    uint32_t pml = 0, nmax = 0;
+    std::vector<int> slots_host;  // per axis: slot_e row, slot_h row, nmax entries each
+    std::vector<float> ind_i0;    // initial inductor currents
+
+    ~Session() {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Every C-ABI entry point runs its body here: a thrown HipError becomes its status code with
+// the message in fg_last_error().
+template <class F>
+int guarded(F &&body) {
+    try {
+        return body();
+    } catch (const HipError &e) {
+        return set_error(e.status, e.msg);
+    } catch (const std::bad_alloc &) {
+        return set_error(FG_TOO_LARGE, "out of host memory");
+    }
+}
+
+// One run's device state: validated, allocated and uploaded, fields at zero (engine.hip).
+Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse, const fg_extras *X);
+// Zero every field, psi, Debye, inductor-current and accumulator buffer and the step count on
+// the session's stream, so the run starts over exactly as from session_new.
+void reset(Session &s);
+// The fused march's blocks of TJ rows x `seg` planes, split by CPML reach: out[0] clear of
+// every slab, out[1] touching only z (k) slabs, out[2] touching an x or y slab.
+void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]);
+
+// Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute a block
+// list so that XCD x receives a contiguous run of the logical order (k tiles fastest, then j
+// tiles): neighbouring tiles then share one L2, and the cache lines their halos and unaligned
+// edges have in common are fetched from HBM once. Padding entries carry `pad`.
+template <class T>
+void xcd_order(std::vector<T> &v, size_t xcds, const T &pad) {
+    const size_t N = v.size(), X = xcds, per = (N + X - 1) / X;
+    if (X <= 1 || N == 0) return;
+    std::vector<T> out(per * X, pad);
+    for (size_t b = 0; b < per * X; b++) {
+        const size_t L = (b % X) * per + b / X;
+        if (L < N) out[b] = v[L];
+    }
+    v.swap(out);
+}
+
+// Sum of E^2 + (376.73 H)^2 over the latest fields (cpu.rs:417-421); synchronizes the stream.
+double energy(Session &s);
+// Copy the run's outputs into a caller-allocated record (run.rs:746-757).
+void read_record(Session &s, fg_record *out);
+
+}  // namespace fg

@@ -5,7 +5,8 @@ Same names, argument meanings and error behaviour as the reference's Rust API:
   Extras, Pulse, Record, run / run_blocking, patches, Patches, Far, time_step, gaussian,
   pulse_shape, idx, unidx, psi_len, and the constants C0, EPS0, MU0, ETA0.
 `Sim.try_new` runs the C++ builder in libfieldgrid.so; `run` runs the HIP engine on an
-MI355X through the C-ABI of include/fieldgrid.h. Errors raise FieldgridError carrying the
+MI355X through the C-ABI of include/fieldgrid.h. Beyond the reference: `run_batch` (many runs
+in shared launches, include/fieldgrid_batch.h) with `BatchRun`, `Tune` and `port_spectra`. Errors raise FieldgridError carrying the
 reference's message ("cancelled", "the FDTD run went unstable", ...).
 
 Two differences of form, not of meaning:
@@ -540,6 +541,115 @@ def run(gpu: Gpu, sim: Sim, driven: int, pulse: Pulse, extras: Extras,
 
 
 run_blocking = run
+
+
+@dataclass
+class BatchRun:
+    """One run of a batched sweep: the arguments of run(), plus the frequencies (Hz) of its
+    port V/I spectra (the sims-antenna post-processing, crates/sims-antenna/src/fdtd/mod.rs:229-246)."""
+    sim: Sim
+    driven: int
+    pulse: Pulse
+    extras: Extras
+    freqs: Sequence[float] = ()
+
+
+@dataclass
+class Tune:
+    """March tile and graph chunk for a batch (0 = autotune), with an optional JSON cache."""
+    tj: int = 0
+    seg: int = 0
+    chunk: int = 0
+    cache: Optional[str] = None
+    used: Tuple[int, int, int] = (0, 0, 0)
+    from_cache: bool = False
+    tune_ms: float = 0.0
+    step_ms: float = 0.0
+
+
+def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
+              progress: Optional[Callable[[int, float], bool]] = None, fields: bool = False):
+    """Run many independent problems together on the HIP engine (fg_run_batch). Each record is
+    what run() gives for that problem. Returns (records, spectra) where spectra[r] is None or
+    (V, I), complex arrays of shape (ports, len(freqs))."""
+    if gpu is None:
+        raise ValueError("run_batch() needs a Gpu")
+    count = len(runs)
+    keep = []
+    problems = (C.POINTER(abi.fg_problem) * max(count, 1))()
+    xs = (C.POINTER(abi.fg_extras) * max(count, 1))()
+    drv = (C.c_uint32 * max(count, 1))()
+    pls = (abi.fg_pulse * max(count, 1))()
+    recs = (abi.fg_record * max(count, 1))()
+    specs = (abi.fg_spectra * max(count, 1))()
+    bufs_all, spec_bufs = [], []
+    for r, job in enumerate(runs):
+        p = job.sim.problem()
+        x = job.extras.to_c(keep)
+        keep += [p, x]
+        problems[r] = C.pointer(p)
+        xs[r] = C.pointer(x)
+        drv[r] = job.driven
+        pls[r] = abi.fg_pulse(job.pulse.f0, job.pulse.fc)
+        rec, bufs = _record_buffers(job.sim, job.extras, fields)
+        recs[r] = rec
+        bufs_all.append(bufs)
+        nf = len(job.freqs)
+        if nf:
+            f = np.ascontiguousarray(job.freqs, np.float64)
+            ports = max(len(job.sim.ports), 1)
+            v = np.zeros(ports * nf * 2, np.float64)
+            i = np.zeros(ports * nf * 2, np.float64)
+            keep += [f, v, i]
+            specs[r] = abi.fg_spectra(nf, 0, f.ctypes.data_as(C.POINTER(C.c_double)),
+                                      v.ctypes.data_as(C.POINTER(C.c_double)), i.ctypes.data_as(C.POINTER(C.c_double)))
+            spec_bufs.append((v, i))
+        else:
+            spec_bufs.append(None)
+    t = abi.fg_tune()
+    if tune is not None:
+        t.tj, t.seg, t.chunk = tune.tj, tune.seg, tune.chunk
+        t.cache = tune.cache.encode() if tune.cache else None
+    errors = []
+
+    def cb(_user, steps, decay):
+        try:
+            return 1 if (progress is None or progress(int(steps), float(decay))) else 0
+        except BaseException as e:
+            errors.append(e)
+            return 0
+
+    fn = abi.PROGRESS_FN(cb)
+    status = abi.lib().fg_run_batch(gpu._h, count, problems, drv, pls, xs, C.byref(t), fn, None, recs, specs)
+    if errors:
+        raise errors[0]
+    abi.check(status)
+    if tune is not None:
+        tune.used = (t.used_tj, t.used_seg, t.used_chunk)
+        tune.from_cache = bool(t.from_cache)
+        tune.tune_ms, tune.step_ms = t.tune_ms, t.step_ms
+    records = [_finish(recs[r], bufs_all[r], runs[r].sim, runs[r].extras) for r in range(count)]
+    spectra = []
+    for r, sb in enumerate(spec_bufs):
+        if sb is None:
+            spectra.append(None)
+            continue
+        nf = len(runs[r].freqs)
+        v, i = (a.reshape(-1, nf, 2) for a in sb)
+        spectra.append((v[..., 0] + 1j * v[..., 1], i[..., 0] + 1j * i[..., 1]))
+    return records, spectra
+
+
+def port_spectra(series: np.ndarray, ports: int, dt: float, freqs: Sequence[float]):
+    """The sims-antenna port spectra (crates/sims-antenna/src/fdtd/mod.rs:231-246) of a record's
+    series, per port, in f64 on the host: V(f) = sum_n V_n dt e^{-iw(n+1)dt},
+    I(f) = sum_n I_n dt e^{-iw(n+1/2)dt}. Returns (V, I) of shape (ports, len(freqs))."""
+    s = np.asarray(series, np.float64).reshape(-1, ports, 2)
+    n = np.arange(s.shape[0], dtype=np.float64)
+    w = 2.0 * np.pi * np.asarray(freqs, np.float64)
+    pe = dt * np.exp(-1j * np.outer(w, n + 1.0) * dt)
+    ph = dt * np.exp(-1j * np.outer(w, n + 0.5) * dt)
+    return (pe @ s[:, :, 0]).T, (ph @ s[:, :, 1]).T
 
 
 class Session:

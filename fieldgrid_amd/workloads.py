@@ -123,6 +123,56 @@ def layered_pulse():
     return F.Pulse(10e9, 10e9)
 
 
+def sweep_runs(count=96, seed=4, lo=48, hi=200, dx=0.5e-3, pml=8, nboxes=8, ndebye=2, steps=(800, 3000),
+               nfreq=64):
+    """The batched sweep: `count` independent runs, each with dims ~ U[lo, hi] per axis, CPML
+    `pml`, `nboxes` dielectric boxes as in boxes() (eps_r ~ U[1,10], sigma ~ U[0,0.05], the first
+    `ndebye` of them Debye with one pole at their median tau), 1-4 single-column 50-ohm z-ports
+    of which one is driven, steps ~ U[steps], and port V/I spectra at `nfreq` frequencies over
+    the 0-20 GHz pulse band. Returns a list of fdtd.BatchRun."""
+    rng = np.random.default_rng(seed)
+    pulse = boxes_pulse()
+    freqs = [(q + 1) * 20e9 / nfreq for q in range(nfreq)]
+    runs = []
+    for _ in range(count):
+        dims = [int(v) for v in rng.integers(lo, hi + 1, size=3)]
+        grid = _grid(dims, dx, pml)
+        n = grid.dims()
+        cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+        mats = F.Materials.new(grid)
+        eps = mats.eps.reshape(cells)
+        sig = mats.sigma.reshape(cells)
+        inf = mats.inf.reshape(cells)
+        dd = mats.debye.reshape(cells)
+        taus = []
+        for b in range(nboxes):
+            a, e = [], []
+            for q in range(3):
+                span = cells[q] - 2 * (pml + 2)
+                size = int(rng.integers(2, max(3, span // 4) + 1))
+                start = int(rng.integers(pml + 2, pml + 2 + max(1, span - size)))
+                a.append(start), e.append(start + size)
+            sl = tuple(slice(a[q], e[q]) for q in range(3))
+            er, s = rng.uniform(1.0, 10.0), rng.uniform(0.0, 0.05)
+            eps[sl], sig[sl], inf[sl] = er, s, er
+            if b < ndebye:
+                dd[sl] = rng.uniform(1.0, 5.0)
+                taus.append(rng.uniform(5e-12, 50e-12))
+        media = F.Media(debye=F.Debye([(1.0 / float(np.median(taus)), 1.0)] if taus else []))
+        nports = int(rng.integers(1, 5))
+        kc = n[2] // 2
+        ports = []
+        for q in range(nports):
+            i = int(rng.integers(pml + 2, n[0] - pml - 2))
+            j = int(rng.integers(pml + 2, n[1] - pml - 2))
+            ports.append(F.PortDef(f"p{q + 1}", [[F.Edge(2, (i, j, k)) for k in range(kc - 2, kc + 2)]], 50.0, False))
+        sim = F.Sim.try_new(grid, mats, None, [], ports, [], media)
+        driven = int(rng.integers(0, nports))
+        k = int(rng.integers(steps[0], steps[1] + 1))
+        runs.append(F.BatchRun(sim, driven, pulse, F.Extras(max_steps=k, decay_db=math.inf), freqs))
+    return runs
+
+
 def bytes_per_node_step(sim, extras=None, core=False):
     """Algorithmic HBM bytes of one step per node: 24 B of E/H read + 24 B written + 6 B of
     u16 coefficient-set indices (12 B when wide) + 8 B per CPML psi element, plus (unless

@@ -1,0 +1,66 @@
+/* fieldgrid batched sweep: many independent FDTD runs on one GPU in shared launches.
+ *
+ * The reference runs one problem per `run()` call (crates/sims-field/src/fdtd/run.rs:390-758);
+ * a port sweep for S-parameters or a parameter sweep is a loop of such calls, each one a
+ * sequence of small dispatches that leaves most of a large GPU idle. fg_run_batch takes the
+ * whole list at once. Each run keeps its own problem, driven port, pulse, extras and record,
+ * with the semantics of fg_run (include/fieldgrid.h): its steps, early stop on decay, and
+ * outputs are those a single fg_run of it gives. It adds port V/I spectra computed on the
+ * device, as the sims-antenna caller does on the host (crates/sims-antenna/src/fdtd/mod.rs:229-266).
+ *
+ * All runs advance together from step 0. Every step is one launch per block class of the fused
+ * march over the blocks of every run still going, plus one launch of the sparse fixes and one
+ * of the observers for all of them. Chunks of steps are captured in a hipGraph and replayed.
+ * The march's tile shape (rows per block, planes per block) and the graph chunk length come
+ * from an autotuner, whose choices can be cached in a JSON file.
+ */
+#ifndef FIELDGRID_BATCH_H
+#define FIELDGRID_BATCH_H
+
+#include "fieldgrid.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tuning of one batch: in/out. Zero asks the autotuner; the choices used come back. */
+typedef struct fg_tune {
+    uint32_t tj;          /* rows per march block, 8 or 14; 0 = tune                           */
+    uint32_t seg;         /* i planes per march block, 1..64; 0 = tune                         */
+    uint32_t chunk;       /* steps per captured hipGraph (rounded up to even); 0 = tune;
+                             1 = no graph (every step launched directly)                       */
+    uint32_t pad;
+    const char *cache;    /* JSON file of tuned choices per batch shape: read if present and
+                             updated after tuning; NULL = no cache                              */
+    /* out */
+    uint32_t used_tj, used_seg, used_chunk, from_cache;
+    double tune_ms;       /* time spent tuning (0 when the cache or the caller decided)         */
+    double step_ms;       /* mean time of one step of the whole batch, as the tuner measured it */
+} fg_tune;
+
+/* Port V/I spectra of one run (crates/sims-antenna/src/fdtd/mod.rs:231-246, per port):
+   V(f) = sum_n V_n dt e^{-i w (n+1) dt},  I(f) = sum_n I_n dt e^{-i w (n+1/2) dt},  in f64. */
+typedef struct fg_spectra {
+    uint32_t nf;          /* number of frequencies                                              */
+    uint32_t pad;
+    const double *freqs;  /* Hz                                                                 */
+    double *v;            /* out: [(port*nf + f)*2 + {re, im}], ports*nf*2 doubles              */
+    double *i;            /* out: same layout                                                   */
+} fg_spectra;
+
+/* Run `count` problems together (a loop of run(), run.rs:390-758, in one batch).
+   problems/driven/pulses/extras/records are arrays of `count`; `spectra` is NULL or an array of
+   `count` (an entry with nf = 0 is skipped). progress(user, steps, decay) is called on the
+   calling thread whenever any run reaches a decay check, with the batch's step count and the
+   smallest decay among the runs still going; returning 0 cancels the whole batch. Resistive
+   sheets and thin wires give FG_UNSUPPORTED, as in fg_run. A run that goes unstable fails the
+   batch with FG_UNSTABLE and "run <r>: the FDTD run went unstable". */
+int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
+                 const fg_pulse *pulses, const fg_extras *const *extras, fg_tune *tune, fg_progress_fn progress,
+                 void *user, fg_record *records, fg_spectra *spectra);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIELDGRID_BATCH_H */

@@ -12,7 +12,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("fieldgrid.h", "fieldgrid_sim.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[a-z_][\w\s\*]*?\b(fg_\w+)\s*\(", text, flags=re.M):
@@ -23,7 +25,7 @@ def declared_functions():
 def test_headers_declare_the_engine_and_builder_entry_points():
     names = declared_functions()
     for must in ("fg_open", "fg_close", "fg_name", "fg_max_cells", "fg_run", "fg_last_error",
-                 "fg_sim_new", "fg_sim_pack", "fg_patches"):
+                 "fg_sim_new", "fg_sim_pack", "fg_patches", "fg_run_batch"):
         assert must in names
 
 
