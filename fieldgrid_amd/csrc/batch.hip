@@ -117,6 +117,8 @@ struct Batch {
     DevBuf<uint32_t> fix_list, obs_list;
     uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0;
     uint32_t tj = 14, seg = 32, chunk = 32, xcds = 8;
+    uint32_t seg_base = 32;  // tuned planes per block; plan() halves it while too few blocks remain
+    uint32_t cus = 256;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> ev;
@@ -142,6 +144,19 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     std::vector<MarchBlock> cls[3];
     std::vector<uint32_t> fl, ol;
     b.fix_gx = b.obs_gx = 0;
+    // as runs finish, fewer blocks remain than the GPU holds: shorter segments (each block
+    // marches fewer planes) keep ~6 blocks per CU in flight. The results do not depend on it.
+    b.seg = b.seg_base;
+    for (;;) {
+        uint64_t blocks = 0;
+        for (uint32_t r = 0; r < b.runs.size(); r++)
+            if (active[r]) {
+                const Session &s = *b.runs[r];
+                blocks += uint64_t((s.n[2] + OWN_K - 1) / OWN_K) * ((s.n[1] + b.tj - 1) / b.tj) * ((s.n[0] + b.seg - 1) / b.seg);
+            }
+        if (blocks >= 6ull * b.cus || b.seg <= 4) break;
+        b.seg /= 2;
+    }
     for (uint32_t r = 0; r < b.runs.size(); r++) {
         if (!active[r]) continue;
         const Session &s = *b.runs[r];
@@ -315,64 +330,63 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
     auto cache = read_cache(t ? t->cache : nullptr);
     const bool ask_tj = !t || !t->tj, ask_seg = !t || !t->seg, ask_chunk = !t || !t->chunk;
     b.tj = t && t->tj ? (t->tj == 8 ? 8 : 14) : 14;
-    b.seg = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
+    b.seg = b.seg_base = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
     b.chunk = t && t->chunk ? t->chunk : 32;
     bool from_cache = false, tuned = false;
     double step_ms = 0.0;
     auto it = cache.find(key);
     if ((ask_tj || ask_seg || ask_chunk) && it != cache.end()) {
         if (ask_tj) b.tj = uint32_t(it->second[0]) == 8 ? 8 : 14;
-        if (ask_seg) b.seg = std::min<uint32_t>(std::max<uint32_t>(uint32_t(it->second[1]), 1), SEG_MAX);
+        if (ask_seg) b.seg = b.seg_base = std::min<uint32_t>(std::max<uint32_t>(uint32_t(it->second[1]), 1), SEG_MAX);
         if (ask_chunk) b.chunk = std::max<uint32_t>(uint32_t(it->second[2]), 1);
         step_ms = it->second.size() > 3 ? it->second[3] : 0.0;
         from_cache = true;
     } else if (ask_tj || ask_seg || ask_chunk) {
         tuned = true;
-        // tile shape: direct launches, so only the kernels' own time differs
+        // tile shape: direct launches, so only the kernels' own time differs. The first
+        // candidate sets how many steps each one gets: about 40 ms of stepping per candidate.
         double best = 1e30;
-        uint32_t btj = b.tj, bseg = b.seg;
+        uint32_t btj = b.tj, bseg = b.seg, steps = 0;
         for (uint32_t tj : {14u, 8u}) {
             if (!ask_tj && tj != b.tj) continue;
             for (uint32_t seg : {32u, 16u, 8u}) {
                 if (!ask_seg && seg != b.seg) continue;
-                b.tj = tj, b.seg = seg;
+                b.tj = tj, b.seg = b.seg_base = seg;
                 plan(b, active);
-                const double ms = time_steps(b, 2, 6, false);
+                if (!steps) steps = std::min<uint32_t>(8, std::max<uint32_t>(2, uint32_t(40.0 / time_steps(b, 1, 1, false))));
+                const double ms = time_steps(b, 1, steps, false);
                 if (ms < best) best = ms, btj = tj, bseg = seg;
             }
         }
-        b.tj = btj, b.seg = bseg;
+        b.tj = btj, b.seg = b.seg_base = bseg;
         plan(b, active);
         step_ms = best;
         if (ask_chunk) {
-            // graph chunk: longer chunks amortise the replay; the batch shrinks as runs end,
-            // so prefer the shortest chunk within 2 % of the best
-            double cbest = 1e30;
-            std::vector<std::pair<uint32_t, double>> got;
-            for (uint32_t c : {8u, 32u, 96u}) {
-                b.chunk = c;
+            // graph chunk: replays only pay off where launches are a visible share of a step
+            // (short steps); there pick the shorter chunk unless the longer one is 2 % faster,
+            // since the batch shrinks as runs end and every shrink recaptures the graph
+            b.chunk = 32;
+            if (best < 1.0) {
+                b.chunk = 8;
                 b.drop_graph();
-                const double ms = time_steps(b, c, c, true);
-                got.push_back({c, ms});
-                cbest = std::min(cbest, ms);
+                const double c8 = time_steps(b, 8, 8, true);
+                b.chunk = 32;
+                b.drop_graph();
+                const double c32 = time_steps(b, 32, 32, true);
+                b.chunk = c32 < 0.98 * c8 ? 32 : 8;
+                step_ms = std::min(step_ms, std::min(c8, c32));
             }
-            for (auto &g : got)
-                if (g.second <= cbest * 1.02) {
-                    b.chunk = g.first;
-                    step_ms = std::min(step_ms, g.second);
-                    break;
-                }
             b.drop_graph();
         }
         FG_CHECK(hipStreamSynchronize(b.stream));
         for (auto &s : b.runs) reset(*s);
-        cache[key] = {double(b.tj), double(b.seg), double(b.chunk), step_ms};
+        cache[key] = {double(b.tj), double(b.seg_base), double(b.chunk), step_ms};
         write_cache(t ? t->cache : nullptr, cache);
     }
     if (b.chunk > 1) b.chunk += b.chunk & 1u;  // replays start on even steps
     plan(b, active);
     if (t) {
-        t->used_tj = b.tj, t->used_seg = b.seg, t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0;
+        t->used_tj = b.tj, t->used_seg = b.seg_base, t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0;
         t->step_ms = step_ms;
         t->tune_ms = tuned ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
     }
@@ -427,6 +441,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         FG_CHECK(hipSetDevice(dev->ordinal));
         FG_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
         b.xcds = dev->cus >= 64 ? uint32_t(dev->cus / 32) : 1u;  // gfx950: 32 CUs per XCD
+        b.cus = uint32_t(std::max(dev->cus, 1));
         std::vector<BatchRun> table(count);
         b.step.alloc(1);
         for (uint32_t r = 0; r < count; r++) {

@@ -170,3 +170,50 @@ def test_decay_stops_a_run_early_like_the_oracle(gpu):
     o = O.run_cpu(sim.desc, 0, W.plain_pulse(), ex, 32, fields=False)
     assert rec.steps == o.steps < 20000
     assert abs(rec.decay_db - o.decay_db) < 0.5
+
+
+def _exact(rec, o32, what):
+    """The device follows the FP32 path operation for operation (no contraction, the pulse and
+    DFT phases from the same f32 expressions): every field value, probe sample and DFT
+    accumulator is bit-identical. Port V/I are f64 sums of f32 products whose order differs
+    (a wave reduction against a sequential loop), so they may differ in the last f32 bit."""
+    for name in ("e", "h", "probes", "plane", "ntff"):
+        a, b = getattr(rec, name), getattr(o32, name)
+        if a is None or b is None or len(b) == 0:
+            continue
+        b = np.asarray(b, np.float32)
+        diff = int(np.sum(a != b))
+        print(f"  {what} {name}: {diff} of {a.size} values differ")
+        assert diff == 0, f"{what} {name}"
+    w = worst(o32.series, rec.series)
+    print(f"  {what} series: worst {w:.2e}")
+    assert w < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
+def test_the_device_reproduces_the_fp32_path_bit_for_bit(gpu, engine):
+    sim, probes = W.plain(nodes=40, pml=6)
+    ex = F.Extras(max_steps=300, decay_db=math.inf, probes=probes, engine=engine)
+    rec = F.run(gpu, sim, 0, W.plain_pulse(), ex, fields=True)
+    _exact(rec, O.run_cpu(sim.desc, 0, W.plain_pulse(), ex, 32), "plain")
+    sim = everything(sheets=False, wires=False)
+    for driven in (0, 1):
+        ex = everything_extras(sim, 600)
+        ex.engine = engine
+        rec = F.run(gpu, sim, driven, F.Pulse(5e9, 5e9), ex, fields=True)
+        _exact(rec, O.run_cpu(sim.desc, driven, F.Pulse(5e9, 5e9), ex, 32), f"everything driven {driven}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_box_tau", [False, True], ids=["global-pole", "pole-per-box"])
+def test_debye_boxes_match_the_oracle(gpu, per_box_tau):
+    """The boxes recipe at 56^3 (CPML 6): dielectric and Debye boxes, one port, NTFF. With
+    pole-per-box each Debye box keeps its own tau through per-edge pole sets (an extension the
+    oracle restates the same way); either way the device is bit-identical to the FP32 path."""
+    sim = W.boxes(nodes=56, pml=6, nboxes=8, ndebye=4, per_box_tau=per_box_tau)
+    ex = W.boxes_extras(sim, steps=400, inset=2)
+    rec = F.run(gpu, sim, 0, W.boxes_pulse(), ex, fields=True)
+    o32 = O.run_cpu(sim.desc, 0, W.boxes_pulse(), ex, 32)
+    assert sim.problem().n_debye > 0
+    _exact(rec, o32, f"boxes {'per-box' if per_box_tau else 'global'}")
