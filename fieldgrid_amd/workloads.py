@@ -5,7 +5,7 @@
                    NTFF box 4 cells inside the CPML at 2, 5, 10 GHz
   layered(...)     768x768x384, 1 mm, CPML 16, eps_r ~ U[1,4] per 8-cell z-layer, two ports
   sweep_runs(...)  the batched sweep's per-run problems (uneven grids, 1-4 ports)
-  design(...)      the adjoint inverse-design problem (192x192x112, design region 64x64x16)
+  design(...)      the adjoint inverse-design problem (192x192x112, design box 64x64x16 cells)
 Every builder takes a size scale so tests can run the same recipe at oracle-sized grids.
 """
 import math
@@ -171,6 +171,50 @@ def sweep_runs(count=96, seed=4, lo=48, hi=200, dx=0.5e-3, pml=8, nboxes=8, ndeb
         k = int(rng.integers(steps[0], steps[1] + 1))
         runs.append(F.BatchRun(sim, driven, pulse, F.Extras(max_steps=k, decay_db=math.inf), freqs))
     return runs
+
+
+def design(dims=(192, 192, 112), dx=0.5e-3, pml=12, box=((64, 128), (64, 128), (48, 64)), port=(30, 96),
+           probe=(162, 96), steps=3000, freq=8e9, seed=5, sigma=0.0):
+    """The inverse-design problem: vacuum with a design box of cells whose eps_r ~ U[1,4] (seed),
+    a 50-ohm single-column z-port at (port[0], port[1]) on one side, driven by a Gaussian pulse
+    f0 = fc = `freq`, and the observed Ez at node (probe[0], probe[1]) at mid-height on the other
+    side. `sigma` > 0 gives the box cells a conductivity ~ U[0, sigma] too. Returns a dict of
+    fdtd.Design arguments plus the objective frequency."""
+    grid = _grid(list(dims), dx, pml)
+    n = grid.dims()
+    cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+    rng = np.random.default_rng(seed)
+    mats = F.Materials.new(grid)
+    lo = tuple(b[0] for b in box)
+    hi = tuple(b[1] for b in box)
+    shape = tuple(h - l for l, h in zip(lo, hi))
+    eps_box = rng.uniform(1.0, 4.0, size=shape).astype(np.float32)
+    sl = tuple(slice(l, h) for l, h in zip(lo, hi))
+    mats.eps.reshape(cells)[sl] = eps_box
+    mats.inf.reshape(cells)[sl] = eps_box
+    if sigma > 0:
+        mats.sigma.reshape(cells)[sl] = rng.uniform(0.0, sigma, size=shape).astype(np.float32)
+    kc = n[2] // 2
+    ports = [F.PortDef("p1", [[F.Edge(2, (port[0], port[1], k)) for k in range(kc - 2, kc + 2)]], 50.0, False)]
+    return dict(grid=grid, mats=mats, pec=None, lumped=[], ports=ports, resistive=[], media=F.Media(),
+                lo=lo, hi=hi, obs=[(2, probe[0], probe[1], kc)], driven=0, pulse=F.Pulse(freq, freq),
+                steps=steps), freq
+
+
+def design_small(steps=300, sigma=0.02, seed=5):
+    """The design recipe at oracle size: 30x28x26 nodes, 1 mm, CPML 5, a 6x5x4-cell box."""
+    return design(dims=(30, 28, 26), dx=1e-3, pml=5, box=((11, 17), (11, 16), (10, 14)), port=(7, 14),
+                  probe=(23, 13), steps=steps, freq=8e9, seed=seed, sigma=sigma)
+
+
+def dft_objective(series, dt, freq):
+    """J = |D|^2 with D = sum_n s_n dt e^{-i w (n+1) dt} (the E-field phase of the reference's
+    running DFTs, cpu.rs:350-353) over one observation's series s_n (E after step n), and
+    dJ/ds_n = 2 Re(conj(D) dt e^{-i w (n+1) dt}). Returns (J, dJ/ds)."""
+    s = np.asarray(series, np.float64).reshape(-1)
+    w = dt * np.exp(-1j * 2.0 * np.pi * freq * (np.arange(s.size) + 1.0) * dt)
+    D = np.sum(s * w)
+    return float(abs(D) ** 2), 2.0 * np.real(np.conj(D) * w)
 
 
 def bytes_per_node_step(sim, extras=None, core=False):

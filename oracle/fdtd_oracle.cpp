@@ -1489,3 +1489,233 @@ int oracle_check(const fg_sim_desc *desc) {
 }
 
 }  // extern "C"
+
+// ---- discrete adjoint of the FP64 step (an extension: the reference has no gradient path) --
+//
+// The observations are point probes of E after every step, s_n = E_c(node) (the point_probes
+// extension above). Given dJ/ds_n the adjoint returns dJ/d eps_r for every cell of the design box
+// [lo, hi). The step is linear in (E, H, psi) and its coefficients depend on eps only through
+// (ca, cb) of the edges whose material average (sim.rs:380-410) reads a design cell:
+//   dJ/dca_e = sum_n lamE_e^{n+1} E_e^n,  dJ/dcb_e = sum_n lamE_e^{n+1} curl_e^n,
+// with lam^{n} = M^T lam^{n+1} + dJ/dx^n. M^T is written here by transposing each statement of
+// update_e (cpu.rs:109-167) and then of update_h (cpu.rs:48-107) in scatter form, statement for
+// statement; the HIP engine uses a gather form derived separately.
+namespace {
+
+struct DesignEdge {
+    size_t c, id;
+    double eps, sig;  // edge averages (F/m, S/m) before the f32 cast
+    size_t cell[4];   // design-box cell index or SIZE_MAX for a cell outside the box
+    double w[4];      // area weight / sum of weights
+};
+
+std::vector<DesignEdge> design_edges(const fg_sim_desc &d, const Sim<double> &sim, const uint32_t lo[3],
+                                     const uint32_t hi[3]) {
+    const size_t *n = sim.n;
+    size_t cells[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
+    for (int a = 0; a < 3; a++)
+        if (lo[a] >= hi[a] || hi[a] > cells[a]) throw Err{"the design box lies outside the grid"};
+    size_t box[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    std::set<std::pair<size_t, size_t>> seen, excluded;
+    for (uint32_t l = 0; l < d.n_lumped; l++)
+        for (uint32_t q = 0; q < d.lumped[l].n_edges; q++) {
+            Edge e = to_edge(d.lumped[l].edges[q]);
+            excluded.insert({e.comp, idx(n, e.at[0], e.at[1], e.at[2])});
+        }
+    for (uint32_t p = 0; p < d.n_ports; p++) {
+        size_t o = 0;
+        for (uint32_t c = 0; c < d.ports[p].n_columns; c++)
+            for (uint32_t q = 0; q < d.ports[p].column_len[c]; q++, o++) {
+                Edge e = to_edge(d.ports[p].edges[o]);
+                excluded.insert({e.comp, idx(n, e.at[0], e.at[1], e.at[2])});
+            }
+    }
+    for (uint32_t r = 0; r < d.n_resistive; r++) excluded.insert({d.resistive[r].comp, size_t(d.resistive[r].id)});
+    std::set<std::pair<size_t, size_t>> debye;
+    for (auto &[c, id, dd, set] : sim.debye_edges) debye.insert({c, id});
+    std::vector<DesignEdge> out;
+    for (size_t ci = lo[0]; ci < hi[0]; ci++)
+        for (size_t cj = lo[1]; cj < hi[1]; cj++)
+            for (size_t ck = lo[2]; ck < hi[2]; ck++)
+                for (size_t c = 0; c < 3; c++) {
+                    size_t u = (c + 1) % 3, v = (c + 2) % 3;
+                    for (size_t du = 0; du < 2; du++)
+                        for (size_t dv = 0; dv < 2; dv++) {
+                            size_t p[3] = {ci, cj, ck};
+                            p[u] += du;
+                            p[v] += dv;
+                            size_t id = idx(n, p[0], p[1], p[2]);
+                            if (!seen.insert({c, id}).second) continue;
+                            if (sim.cb[c][id] == 0.0 || debye.count({c, id})) continue;  // eps does not enter
+                            if (excluded.count({c, id}))
+                                throw Err{"the design box touches a port, lumped or resistive edge"};
+                            DesignEdge e{c, id, 0, 0, {}, {}};
+                            double sum_e = 0, sum_s = 0, w_sum = 0;
+                            const long offs[4][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}};
+                            for (int q = 0; q < 4; q++) {
+                                long cell[3] = {long(p[0]), long(p[1]), long(p[2])};
+                                cell[u] += offs[q][0];
+                                cell[v] += offs[q][1];
+                                double w = sim.ax[u].d[size_t(cell[u])] * sim.ax[v].d[size_t(cell[v])];
+                                size_t cid = (size_t(cell[0]) * cells[1] + size_t(cell[1])) * cells[2] + size_t(cell[2]);
+                                sum_e += (d.eps ? double(d.eps[cid]) : 1.0) * w;
+                                sum_s += (d.sigma ? double(d.sigma[cid]) : 0.0) * w;
+                                w_sum += w;
+                                bool in = true;
+                                for (int a = 0; a < 3; a++) in = in && size_t(cell[a]) >= lo[a] && size_t(cell[a]) < hi[a];
+                                e.cell[q] = in ? ((size_t(cell[0]) - lo[0]) * box[1] + (size_t(cell[1]) - lo[1])) * box[2] +
+                                                     (size_t(cell[2]) - lo[2])
+                                               : SIZE_MAX;
+                                e.w[q] = w;
+                            }
+                            for (int q = 0; q < 4; q++) e.w[q] /= w_sum;
+                            e.eps = EPS0 * sum_e / w_sum;
+                            e.sig = sum_s / w_sum;
+                            out.push_back(e);
+                        }
+                }
+    return out;
+}
+
+int design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pulse, uint64_t steps, uint32_t n_obs,
+                    const fg_point_probe *obs, const uint32_t lo[3], const uint32_t hi[3], const double *dseries,
+                    double *grad, double *series_out) {
+    Sim<double> sim = build<double>(*desc);
+    if (!sim.inductors.empty() || !sim.sheets.empty() || !sim.wires.empty() || !sim.debye_edges.empty())
+        throw Err{"the adjoint covers curl updates, CPML and resistive ports only"};
+    for (uint32_t q = 0; q < n_obs; q++)
+        if (obs[q].field > 2) throw Err{"observations are E components"};
+    fg_extras ex{};
+    ex.max_steps = steps;
+    ex.decay_db = INFINITY;
+    ex.plane_k = -1;
+    ex.n_probes = n_obs;
+    ex.probes = obs;
+    Packed<double> pk = pack(sim, driven, *pulse, ex);
+    State<double> st(sim, std::move(pk), ex, *pulse);
+    const std::vector<DesignEdge> de = design_edges(*desc, sim, lo, hi);
+    const size_t ne = de.size(), nn = sim.nn;
+    const size_t *n = sim.n;
+    std::vector<double> tape_e(steps * ne), tape_c(steps * ne);
+    for (size_t s = 0; s < steps; s++) {
+        for (size_t q = 0; q < ne; q++) tape_e[s * ne + q] = st.e[de[q].c * nn + de[q].id];
+        st.step(s);
+        for (size_t q = 0; q < ne; q++) {  // E^{n+1} = ca E^n + cb curl^n on these edges
+            size_t f = de[q].c * nn + de[q].id;
+            tape_c[s * ne + q] = (st.e[f] - sim.ca[de[q].c][de[q].id] * tape_e[s * ne + q]) / sim.cb[de[q].c][de[q].id];
+        }
+    }
+    if (series_out)
+        for (size_t q = 0; q < steps * n_obs; q++) series_out[q] = st.probes[q];
+    std::vector<double> le(3 * nn, 0.0), lh(3 * nn, 0.0), lp(st.psi.size(), 0.0);
+    std::vector<double> gca(ne, 0.0), gcb(ne, 0.0);
+    const Axis<double> *ax = sim.ax;
+    const size_t pml = sim.pml;
+    const double k_mu = sim.dt / MU0;
+    for (size_t s = steps; s-- > 0;) {
+        for (uint32_t q = 0; q < n_obs; q++)  // s_n = E after step n
+            le[obs[q].field * nn + idx(n, obs[q].i, obs[q].j, obs[q].k)] += dseries[s * n_obs + q];
+        for (size_t q = 0; q < ne; q++) {
+            double l = le[de[q].c * nn + de[q].id];
+            gca[q] += l * tape_e[s * ne + q];
+            gcb[q] += l * tape_c[s * ne + q];
+        }
+        // transpose of update_e: E_c = ca E_c + cb (t1 - t2), t1 = hv inv_dd_u + psi_u', ...
+        for (size_t i = 0; i < n[0]; i++)
+            for (size_t j = 0; j < n[1]; j++)
+                for (size_t k = 0; k < n[2]; k++) {
+                    size_t pp[3] = {i, j, k};
+                    size_t id = idx(n, i, j, k);
+                    for (size_t c = 0; c < 3; c++) {
+                        double cb = sim.cb[c][id];
+                        if (cb == 0.0) continue;
+                        size_t u = (c + 1) % 3, v = (c + 2) % 3;
+                        double a = cb * le[c * nn + id];
+                        le[c * nn + id] *= sim.ca[c][id];
+                        size_t pu = pp[u], pv = pp[v];
+                        double hv = a * ax[u].inv_dd[pu], hu = -a * ax[v].inv_dd[pv];
+                        if (pml > 0) {
+                            if (ax[u].slot_e[pu] >= 0) {
+                                double &r = lp[st.base(0, c, u) + psi_idx(n, u, pml, pp, size_t(ax[u].slot_e[pu]))];
+                                double tot = r + a;  // adjoint of psi_u' (later steps + this t1)
+                                hv += ax[u].ce[pu] * ax[u].raw_inv_dd[pu] * tot;
+                                r = ax[u].be[pu] * tot;
+                            }
+                            if (ax[v].slot_e[pv] >= 0) {
+                                double &r = lp[st.base(0, c, v) + psi_idx(n, v, pml, pp, size_t(ax[v].slot_e[pv]))];
+                                double tot = r - a;
+                                hu += ax[v].ce[pv] * ax[v].raw_inv_dd[pv] * tot;
+                                r = ax[v].be[pv] * tot;
+                            }
+                        }
+                        size_t um[3] = {i, j, k}, vm[3] = {i, j, k};
+                        um[u] -= 1;
+                        vm[v] -= 1;
+                        lh[v * nn + id] += hv;  // hv = H_v(p) - H_v(p - u)
+                        lh[v * nn + idx(n, um[0], um[1], um[2])] -= hv;
+                        lh[u * nn + id] += hu;  // hu = H_u(p) - H_u(p - v)
+                        lh[u * nn + idx(n, vm[0], vm[1], vm[2])] -= hu;
+                    }
+                }
+        // transpose of update_h: H_c -= k_mu (t1 - t2), t1 = ev inv_d_u + psi_u', ...
+        for (size_t i = 0; i < n[0]; i++)
+            for (size_t j = 0; j < n[1]; j++)
+                for (size_t k = 0; k < n[2]; k++) {
+                    size_t pp[3] = {i, j, k};
+                    size_t id = idx(n, i, j, k);
+                    for (size_t c = 0; c < 3; c++) {
+                        size_t u = (c + 1) % 3, v = (c + 2) % 3;
+                        if (pp[u] + 1 >= n[u] || pp[v] + 1 >= n[v]) continue;
+                        double b = -k_mu * lh[c * nn + id];
+                        size_t pu = pp[u], pv = pp[v];
+                        double ev = b * ax[u].inv_d[pu], eu = -b * ax[v].inv_d[pv];
+                        if (pml > 0) {
+                            if (ax[u].slot_h[pu] >= 0) {
+                                double &r = lp[st.base(1, c, u) + psi_idx(n, u, pml, pp, size_t(ax[u].slot_h[pu]))];
+                                double tot = r + b;
+                                ev += ax[u].ch[pu] * ax[u].raw_inv_d[pu] * tot;
+                                r = ax[u].bh[pu] * tot;
+                            }
+                            if (ax[v].slot_h[pv] >= 0) {
+                                double &r = lp[st.base(1, c, v) + psi_idx(n, v, pml, pp, size_t(ax[v].slot_h[pv]))];
+                                double tot = r - b;
+                                eu += ax[v].ch[pv] * ax[v].raw_inv_d[pv] * tot;
+                                r = ax[v].bh[pv] * tot;
+                            }
+                        }
+                        size_t up[3] = {i, j, k}, vp[3] = {i, j, k};
+                        up[u] += 1;
+                        vp[v] += 1;
+                        le[v * nn + idx(n, up[0], up[1], up[2])] += ev;  // ev = E_v(p + u) - E_v(p)
+                        le[v * nn + id] -= ev;
+                        le[u * nn + idx(n, vp[0], vp[1], vp[2])] += eu;  // eu = E_u(p + v) - E_u(p)
+                        le[u * nn + id] -= eu;
+                    }
+                }
+    }
+    // chain rule: ca = (1-x)/(1+x), cb = dt/(eps (1+x)), x = sig dt / (2 eps) (sim.rs:431-441)
+    size_t box = size_t(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    for (size_t q = 0; q < box; q++) grad[q] = 0.0;
+    const double dt = sim.dt;
+    for (size_t q = 0; q < ne; q++) {
+        const double eps = de[q].eps, x = de[q].sig * dt / (2.0 * eps);
+        const double dca = 2.0 * x / (eps * (1.0 + x) * (1.0 + x)), dcb = -dt / (eps * eps * (1.0 + x) * (1.0 + x));
+        const double g = gca[q] * dca + gcb[q] * dcb;  // dJ / d eps_edge (F/m)
+        for (int t = 0; t < 4; t++)
+            if (de[q].cell[t] != SIZE_MAX) grad[de[q].cell[t]] += g * EPS0 * de[q].w[t];
+    }
+    return FG_OK;
+}
+
+}  // namespace
+
+extern "C" int oracle_design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pulse, uint64_t steps,
+                                      uint32_t n_obs, const fg_point_probe *obs, const uint32_t lo[3], const uint32_t hi[3],
+                                      const double *dseries, double *grad, double *series_out) {
+    try {
+        return design_gradient(desc, driven, pulse, steps, n_obs, obs, lo, hi, dseries, grad, series_out);
+    } catch (const Err &e) {
+        g_err = e.msg;
+        return FG_INVALID;
+    }
+}

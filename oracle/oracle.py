@@ -57,6 +57,10 @@ def lib():
         L.oracle_threads.restype = C.c_int
         L.oracle_check.restype = C.c_int
         L.oracle_check.argtypes = [P(abi.fg_sim_desc)]
+        L.oracle_design_gradient.restype = C.c_int
+        L.oracle_design_gradient.argtypes = [P(abi.fg_sim_desc), C.c_uint32, P(abi.fg_pulse), C.c_uint64, C.c_uint32,
+                                             P(abi.fg_point_probe), P(C.c_uint32), P(C.c_uint32), P(C.c_double),
+                                             P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -192,3 +196,24 @@ def threads() -> int:
 def set_mutation(m: int):
     """Plant mistake number m in the oracle (0 = none). Tests only."""
     lib().oracle_set_mutation(m)
+
+
+def design_gradient(desc, driven, pulse, steps, obs, lo, hi, dseries):
+    """The FP64 discrete adjoint: dJ/d eps_r over the design box [lo, hi) of cells (k fastest),
+    given dJ/ds_n for the E observations `obs` = [(field, i, j, k)] after every step.
+    Returns (grad shaped like the box, series (steps, n_obs))."""
+    n_obs = len(obs)
+    pr = (abi.fg_point_probe * max(n_obs, 1))(*[abi.fg_point_probe(*o) for o in obs])
+    lo_ = (C.c_uint32 * 3)(*lo)
+    hi_ = (C.c_uint32 * 3)(*hi)
+    box = tuple(int(h - l) for l, h in zip(lo, hi))
+    grad = np.zeros(box)
+    series = np.zeros((steps, max(n_obs, 1)))
+    ds = np.ascontiguousarray(dseries, np.float64).reshape(-1)
+    assert ds.size == steps * n_obs
+    pl = abi.fg_pulse(pulse.f0, pulse.fc)
+    st = lib().oracle_design_gradient(
+        C.byref(desc.desc), driven, C.byref(pl), steps, n_obs, pr, lo_, hi_, _dptr(ds), _dptr(grad), _dptr(series))
+    if st != 0:
+        raise OracleError(st, lib().oracle_last_error().decode())
+    return grad, series[:, :n_obs]

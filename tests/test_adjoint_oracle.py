@@ -1,0 +1,89 @@
+"""The FP64 discrete adjoint of the oracle (oracle/fdtd_oracle.cpp, design_gradient), pinned by
+central finite differences of the FP64 oracle's own forward runs.
+
+The objective is J = |sum_n Ez_n dt e^{-iw(n+1)dt}|^2 at an observation node (8 GHz), the
+design variables eps_r of a box of cells; the box cells also carry a conductivity, so both
+dca/deps and dcb/deps enter. The adjoint wave reaches the CPML, so its psi transposition is
+exercised too. h = 1e-3 in eps_r (rounded to the f32 cell value; the difference quotient uses the
+rounded step). FP64 central differences are accurate to ~1e-7 here, so the bar is 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fieldgrid_amd import fdtd as F
+from fieldgrid_amd import workloads as W
+from oracle import oracle as O
+
+
+def objective64(args, freq, eps_box=None):
+    mats = args["mats"]
+    n = args["grid"].dims()
+    cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+    sl = tuple(slice(l, h) for l, h in zip(args["lo"], args["hi"]))
+    saved = mats.eps.copy()
+    if eps_box is not None:
+        mats.eps.reshape(cells)[sl] = eps_box
+    try:
+        desc = F.SimDesc(args["grid"], mats, args["pec"], args["lumped"], args["ports"], args["resistive"], args["media"])
+        ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+        r = O.run_cpu(desc, args["driven"], args["pulse"], ex, 64, fields=False)
+        return W.dft_objective(r.probes, r.dt, freq)[0], r.dt
+    finally:
+        mats.eps[:] = saved
+
+
+def test_the_fp64_adjoint_matches_central_differences():
+    args, freq = W.design_small(steps=260)
+    grid, mats = args["grid"], args["mats"]
+    desc = F.SimDesc(grid, mats, args["pec"], args["lumped"], args["ports"], args["resistive"], args["media"])
+    ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+    fwd = O.run_cpu(desc, args["driven"], args["pulse"], ex, 64, fields=False)
+    J, dJ = W.dft_objective(fwd.probes, fwd.dt, freq)
+    grad, series = O.design_gradient(desc, args["driven"], args["pulse"], args["steps"], args["obs"],
+                                     args["lo"], args["hi"], dJ)
+    assert np.array_equal(series[:, 0], fwd.probes)  # the adjoint's forward is the oracle's run
+    n = grid.dims()
+    cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+    sl = tuple(slice(l, h) for l, h in zip(args["lo"], args["hi"]))
+    box = mats.eps.reshape(cells)[sl].copy()
+    rng = np.random.default_rng(11)
+    picks = [tuple(int(rng.integers(0, s)) for s in box.shape) for _ in range(6)]
+    h = 1e-3
+    fds = []
+    for c in picks:
+        up, dn = box.copy(), box.copy()
+        up[c] = np.float32(box[c] + h)
+        dn[c] = np.float32(box[c] - h)
+        jp, _ = objective64(args, freq, up)
+        jm, _ = objective64(args, freq, dn)
+        fds.append((jp - jm) / (float(up[c]) - float(dn[c])))
+    fds = np.array(fds)
+    got = np.array([grad[c] for c in picks])
+    scale = np.max(np.abs(fds))
+    print(f"\nJ = {J:.6e}; adjoint {got}\n  central differences {fds}")
+    assert scale > 0
+    assert np.max(np.abs(got - fds)) < 1e-5 * scale
+
+
+def test_a_wrong_adjoint_sign_would_be_caught():
+    """The finite-difference check is sharp: the gradient of the objective with the observation
+    moved one node (a plausible indexing slip) differs from the true one by far more than 1e-5."""
+    args, freq = W.design_small(steps=260)
+    desc = F.SimDesc(args["grid"], args["mats"], None, [], args["ports"], [], args["media"])
+    ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+    fwd = O.run_cpu(desc, 0, args["pulse"], ex, 64, fields=False)
+    _, dJ = W.dft_objective(fwd.probes, fwd.dt, freq)
+    g, _ = O.design_gradient(desc, 0, args["pulse"], args["steps"], args["obs"], args["lo"], args["hi"], dJ)
+    f, i, j, k = args["obs"][0]
+    g2, _ = O.design_gradient(desc, 0, args["pulse"], args["steps"], [(f, i, j, k + 1)], args["lo"], args["hi"], dJ)
+    assert np.max(np.abs(g - g2)) > 1e-2 * np.max(np.abs(g))
+
+
+def test_the_design_box_may_not_touch_a_port():
+    args, _ = W.design_small(steps=10)
+    desc = F.SimDesc(args["grid"], args["mats"], None, [], args["ports"], [], args["media"])
+    lo = (5, 12, 10)  # the port column sits at (7, 14)
+    with pytest.raises(O.OracleError, match="port"):
+        O.design_gradient(desc, 0, args["pulse"], 10, args["obs"], lo, args["hi"], np.zeros(10))
