@@ -131,6 +131,11 @@ class fg_spectra(C.Structure):
     _fields_ = [("nf", u32), ("pad", u32), ("freqs", P(f64)), ("v", P(f64)), ("i", P(f64))]
 
 
+class fg_design(C.Structure):
+    _fields_ = [("lo", u32 * 3), ("hi", u32 * 3), ("n_obs", u32), ("every", u32), ("obs", P(fg_point_probe)),
+                ("steps", u64)]
+
+
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, u64, f64)
 
 # every symbol the headers declare, with its ctypes signature
@@ -163,6 +168,12 @@ SIGNATURES = {
     "fg_patches": (u64, [C.c_void_p, u32, P(u32), P(f64), P(f64)]),
     "fg_run_batch": (C.c_int, [C.c_void_p, u32, P(P(fg_problem)), P(u32), P(fg_pulse), P(P(fg_extras)),
                                P(fg_tune), PROGRESS_FN, C.c_void_p, P(fg_record), P(fg_spectra)]),
+    "fg_adjoint_new": (C.c_int, [C.c_void_p, P(fg_sim_desc), P(fg_design), u32, P(fg_pulse), P(C.c_void_p)]),
+    "fg_adjoint_forward": (C.c_int, [C.c_void_p, P(f32), P(f32)]),
+    "fg_adjoint_backward": (C.c_int, [C.c_void_p, P(f64), P(f64)]),
+    "fg_adjoint_forward64": (C.c_int, [C.c_void_p, P(f64), P(f64)]),
+    "fg_adjoint_timing": (C.c_int, [C.c_void_p, P(f64), P(u64)]),
+    "fg_adjoint_free": (None, [C.c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,686 @@
+// Discrete adjoint behind include/fieldgrid_adjoint.h: dJ/d eps_r over a design box.
+//
+// No counterpart in the reference. One forward step (cpu.rs:267-307, the fused engine here)
+// maps the state x = (E, H, psi) linearly: the H update (cpu.rs:48-107) and then the E update
+// (cpu.rs:109-167), whose coefficients (ca, cb) are the only place eps enters. The adjoint state
+// lam = (lamE, lamH, lamPsi) runs backward:
+//   lam^n = M^T lam^{n+1} + dJ/dx^n,  dJ/dca_e += lamE_e^{n+1} E_e^n,  dJ/dcb_e += lamE_e^{n+1} curl_e^n.
+// M^T is written in gather form, so each thread owns its outputs:
+//   k_adj_e  transposes the E update: H-bar gathers the "a = cb lamE" terms of the E edges whose
+//            curl reads it, at the node and its +1 neighbours (the shape of an H update);
+//   k_adj_h  transposes the H update and finishes lamE = ca lamE + (the "b = -k_mu lamH" terms
+//            of the H components whose curl reads it, at the node and its -1 neighbours), and
+//            advances every CPML psi adjoint into the other ping-pong copy.
+// The forward pass keeps field snapshots every `every` steps; the backward pass restores one,
+// recomputes that segment while taping (E^n, curl^n) on the design edges, then runs the adjoint
+// over the segment. An f64 forward (k_h64 / k_e64) evaluates the same problem with the box's
+// edge coefficients in f64, for finite-difference checks.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fieldgrid_adjoint.h"
+#include "host.h"
+
+namespace fg {
+
+// hv / hu adjoint terms of E edge C at node (i,j,k): with a = cb lamE, the E update's
+// t1 = hv inv_dd_u + psi_u', t2 = hu inv_dd_v + psi_v', psi' = be psi + ce (h diff) raw_dd gives
+//   hv-bar = a inv_dd_u + ce_u raw_dd_u (psiBar_u + a),  hu-bar = -a inv_dd_v + ce_v raw_dd_v (psiBar_v - a).
+// Zero outside the grid and where cb = 0 (the edge is never updated).
+template <int C>
+__device__ __forceinline__ void e_terms(const Geo &g, const float *__restrict__ lamE, const float *__restrict__ lp,
+                                        uint32_t i, uint32_t j, uint32_t k, float &hv, float &hu) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    hv = 0.f, hu = 0.f;
+    if (i >= g.n0 || j >= g.n1 || k >= g.n2) return;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const float cb = coef_at(g, C * g.nn + id).y;
+    if (cb == 0.f) return;
+    const float a = cb * lamE[C * g.nn + id];
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    hv = a * g.ax[U].inv_dd[pu];
+    hu = -a * g.ax[V].inv_dd[pv];
+    if (g.pml) {
+        const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
+        if (su >= 0) hv += g.ax[U].ce[pu] * g.ax[U].raw_dd[pu] * (lp[psi_at<U>(g, 0, C, i, j, k, uint32_t(su))] + a);
+        if (sv >= 0) hu += g.ax[V].ce[pv] * g.ax[V].raw_dd[pv] * (lp[psi_at<V>(g, 0, C, i, j, k, uint32_t(sv))] - a);
+    }
+}
+
+// ev / eu adjoint terms of H component C at node (i,j,k): with b = -k_mu lamH, the H update's
+// t1 = ev inv_d_u + psi_u', t2 = eu inv_d_v + psi_v', psi' = bh psi + ch (e diff) raw_d gives
+//   ev-bar = b inv_d_u + ch_u raw_d_u (psiBar_u + b),  eu-bar = -b inv_d_v + ch_v raw_d_v (psiBar_v - b).
+// Zero outside the grid and for components the H update skips.
+template <int C>
+__device__ __forceinline__ void h_terms(const Geo &g, const float *__restrict__ lamH, const float *__restrict__ lp,
+                                        uint32_t i, uint32_t j, uint32_t k, float &ev, float &eu) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    ev = 0.f, eu = 0.f;
+    if (i >= g.n0 || j >= g.n1 || k >= g.n2) return;
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    if (pu + 1 >= dim<U>(g) || pv + 1 >= dim<V>(g)) return;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const float b = -g.k_mu * lamH[C * g.nn + id];
+    ev = b * g.ax[U].inv_d[pu];
+    eu = -b * g.ax[V].inv_d[pv];
+    if (g.pml) {
+        const int su = g.ax[U].slot_h[pu], sv = g.ax[V].slot_h[pv];
+        if (su >= 0) ev += g.ax[U].ch[pu] * g.ax[U].raw_d[pu] * (lp[psi_at<U>(g, 1, C, i, j, k, uint32_t(su))] + b);
+        if (sv >= 0) eu += g.ax[V].ch[pv] * g.ax[V].raw_d[pv] * (lp[psi_at<V>(g, 1, C, i, j, k, uint32_t(sv))] - b);
+    }
+}
+
+// Transpose of the E update: lamH += sum of the hv/hu terms of the E edges that read each H.
+//   Ex reads Hz(p)-Hz(p-y), Hy(p)-Hy(p-z); Ey reads Hx(p)-Hx(p-z), Hz(p)-Hz(p-x);
+//   Ez reads Hy(p)-Hy(p-x), Hx(p)-Hx(p-y)
+__global__ __launch_bounds__(256) void k_adj_e(Geo g, const float *__restrict__ lamE, float *__restrict__ lamH,
+                                               const float *__restrict__ lp) {
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= g.n2 || j >= g.n1) return;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    float hv0, hu0, hv1, hu1, hv2, hu2, t, x0y, x0z, x1z, x1x, x2x, x2y;
+    e_terms<0>(g, lamE, lp, i, j, k, hv0, hu0);
+    e_terms<1>(g, lamE, lp, i, j, k, hv1, hu1);
+    e_terms<2>(g, lamE, lp, i, j, k, hv2, hu2);
+    e_terms<0>(g, lamE, lp, i, j + 1, k, x0y, t);  // hv of Ex at p+y
+    e_terms<0>(g, lamE, lp, i, j, k + 1, t, x0z);  // hu of Ex at p+z
+    e_terms<1>(g, lamE, lp, i, j, k + 1, x1z, t);  // hv of Ey at p+z
+    e_terms<1>(g, lamE, lp, i + 1, j, k, t, x1x);  // hu of Ey at p+x
+    e_terms<2>(g, lamE, lp, i + 1, j, k, x2x, t);  // hv of Ez at p+x
+    e_terms<2>(g, lamE, lp, i, j + 1, k, t, x2y);  // hu of Ez at p+y
+    lamH[id] += (hv1 - x1z) + (hu2 - x2y);
+    lamH[g.nn + id] += (hv2 - x2x) + (hu0 - x0z);
+    lamH[2 * g.nn + id] += (hv0 - x0y) + (hu1 - x1x);
+}
+
+// psi adjoint of the node's own slots: psiBar_pre = be (psiBar + s) for an advanced slot, carried
+// over unchanged where the forward step left the psi value alone.
+template <int C>
+__device__ __forceinline__ void psi_e_own(const Geo &g, float a, bool on, const float *__restrict__ lp,
+                                          float *__restrict__ ln, uint32_t i, uint32_t j, uint32_t k) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
+    if (su >= 0) {
+        const uint64_t q = psi_at<U>(g, 0, C, i, j, k, uint32_t(su));
+        ln[q] = on ? g.ax[U].be[pu] * (lp[q] + a) : lp[q];
+    }
+    if (sv >= 0) {
+        const uint64_t q = psi_at<V>(g, 0, C, i, j, k, uint32_t(sv));
+        ln[q] = on ? g.ax[V].be[pv] * (lp[q] - a) : lp[q];
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void psi_h_own(const Geo &g, const float *__restrict__ lamH, const float *__restrict__ lp,
+                                          float *__restrict__ ln, uint32_t i, uint32_t j, uint32_t k) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    const bool on = pu + 1 < dim<U>(g) && pv + 1 < dim<V>(g);
+    const float b = -g.k_mu * lamH[C * g.nn + (uint64_t(i) * g.n1 + j) * g.n2 + k];
+    const int su = g.ax[U].slot_h[pu], sv = g.ax[V].slot_h[pv];
+    if (su >= 0) {
+        const uint64_t q = psi_at<U>(g, 1, C, i, j, k, uint32_t(su));
+        ln[q] = on ? g.ax[U].bh[pu] * (lp[q] + b) : lp[q];
+    }
+    if (sv >= 0) {
+        const uint64_t q = psi_at<V>(g, 1, C, i, j, k, uint32_t(sv));
+        ln[q] = on ? g.ax[V].bh[pv] * (lp[q] - b) : lp[q];
+    }
+}
+
+// Transpose of the H update, and the rest of the E update's: lamE = ca lamE + the ev/eu terms of
+// the H components that read each E (Hx reads Ez(p+y)-Ez(p), Ey(p+z)-Ey(p); Hy reads
+// Ex(p+z)-Ex(p), Ez(p+x)-Ez(p); Hz reads Ey(p+x)-Ey(p), Ex(p+y)-Ex(p)); every psi adjoint
+// advances from lp into ln.
+__global__ __launch_bounds__(256) void k_adj_h(Geo g, float *__restrict__ lamE, const float *__restrict__ lamH,
+                                               const float *__restrict__ lp, float *__restrict__ ln) {
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= g.n2 || j >= g.n1) return;
+    const uint64_t nn = g.nn, id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    float ev0, eu0, ev1, eu1, ev2, eu2, t, y0y, y0z, y1z, y1x, y2x, y2y;
+    h_terms<0>(g, lamH, lp, i, j, k, ev0, eu0);
+    h_terms<1>(g, lamH, lp, i, j, k, ev1, eu1);
+    h_terms<2>(g, lamH, lp, i, j, k, ev2, eu2);
+    // -1 neighbours (an index of -1 wraps past the grid and reads as zero)
+    h_terms<0>(g, lamH, lp, i, j - 1, k, y0y, t);  // ev of Hx at p-y  (Hx: ev on Ez, u = y)
+    h_terms<0>(g, lamH, lp, i, j, k - 1, t, y0z);  // eu of Hx at p-z  (Hx: eu on Ey, v = z)
+    h_terms<1>(g, lamH, lp, i, j, k - 1, y1z, t);  // ev of Hy at p-z  (Hy: ev on Ex, u = z)
+    h_terms<1>(g, lamH, lp, i - 1, j, k, t, y1x);  // eu of Hy at p-x  (Hy: eu on Ez, v = x)
+    h_terms<2>(g, lamH, lp, i - 1, j, k, y2x, t);  // ev of Hz at p-x  (Hz: ev on Ey, u = x)
+    h_terms<2>(g, lamH, lp, i, j - 1, k, t, y2y);  // eu of Hz at p-y  (Hz: eu on Ex, v = y)
+    float le[3] = {lamE[id], lamE[nn + id], lamE[2 * nn + id]};
+    float a[3];
+    bool on[3];
+    for (int c = 0; c < 3; c++) {
+        const float2 cf = coef_at(g, c * nn + id);
+        on[c] = cf.y != 0.f;
+        a[c] = cf.y * le[c];
+        if (on[c]) le[c] *= cf.x;
+    }
+    lamE[id] = le[0] + (y1z - ev1) + (y2y - eu2);
+    lamE[nn + id] = le[1] + (y2x - ev2) + (y0z - eu0);
+    lamE[2 * nn + id] = le[2] + (y0y - ev0) + (y1x - eu1);
+    if (g.pml) {
+        psi_e_own<0>(g, a[0], on[0], lp, ln, i, j, k);
+        psi_e_own<1>(g, a[1], on[1], lp, ln, i, j, k);
+        psi_e_own<2>(g, a[2], on[2], lp, ln, i, j, k);
+        psi_h_own<0>(g, lamH, lp, ln, i, j, k);
+        psi_h_own<1>(g, lamH, lp, ln, i, j, k);
+        psi_h_own<2>(g, lamH, lp, ln, i, j, k);
+    }
+}
+
+// dJ/ds_n enters lamE at the observed edges (s_n is E after step n)
+__global__ void k_adj_inject(float *__restrict__ lamE, const uint64_t *__restrict__ obs, uint32_t n_obs,
+                             const double *__restrict__ ds) {
+    const uint32_t q = threadIdx.x;
+    if (q < n_obs) lamE[obs[q]] += float(ds[q]);
+}
+
+// The forward step's E^n and curl^n = (t1 - t2) on the design edges (no CPML there), from the
+// previous E copy and the new H copy of the ping-pong pair.
+__global__ __launch_bounds__(256) void k_adj_tape(Geo g, const uint64_t *__restrict__ edges, uint32_t ne,
+                                                  const float *__restrict__ Eo, const float *__restrict__ Hn,
+                                                  float2 *__restrict__ tape) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const uint64_t f = edges[e], nn = g.nn;
+    const uint32_t c = uint32_t(f / nn);
+    const uint64_t id = f - c * nn;
+    const uint32_t k = uint32_t(id % g.n2), j = uint32_t((id / g.n2) % g.n1), i = uint32_t(id / (uint64_t(g.n1) * g.n2));
+    const uint64_t sx = uint64_t(g.n1) * g.n2, sy = g.n2;
+    float curl;
+    if (c == 0) {  // u = y, v = z: hv = Hz(p)-Hz(p-y), hu = Hy(p)-Hy(p-z)
+        curl = (Hn[2 * nn + id] - Hn[2 * nn + id - sy]) * g.ax[1].inv_dd[j] - (Hn[nn + id] - Hn[nn + id - 1]) * g.ax[2].inv_dd[k];
+    } else if (c == 1) {  // u = z, v = x: hv = Hx(p)-Hx(p-z), hu = Hz(p)-Hz(p-x)
+        curl = (Hn[id] - Hn[id - 1]) * g.ax[2].inv_dd[k] - (Hn[2 * nn + id] - Hn[2 * nn + id - sx]) * g.ax[0].inv_dd[i];
+    } else {  // u = x, v = y: hv = Hy(p)-Hy(p-x), hu = Hx(p)-Hx(p-y)
+        curl = (Hn[nn + id] - Hn[nn + id - sx]) * g.ax[0].inv_dd[i] - (Hn[id] - Hn[id - sy]) * g.ax[1].inv_dd[j];
+    }
+    tape[e] = make_float2(Eo[f], curl);
+}
+
+__global__ __launch_bounds__(256) void k_adj_grad(const uint64_t *__restrict__ edges, uint32_t ne,
+                                                  const float *__restrict__ lamE, const float2 *__restrict__ tape,
+                                                  double *__restrict__ gca, double *__restrict__ gcb) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const double l = lamE[edges[e]];
+    const float2 t = tape[e];
+    gca[e] += l * double(t.x);
+    gcb[e] += l * double(t.y);
+}
+
+// ---- f64 forward (finite-difference checks) -------------------------------------------------
+
+struct Geo64 {
+    Geo g;                   // grid, axes (f32 tables, promoted), psi offsets
+    const uint32_t *idx;     // one u32 set index per edge
+    const double2 *sets;     // (ca, cb) in f64
+    double k_mu;
+};
+
+template <int C>
+__device__ __forceinline__ double h64(const Geo64 &G, const double *__restrict__ E, const double *__restrict__ H,
+                                      double *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
+    const Geo &g = G.g;
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const double hold = H[C * g.nn + id];
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    if (pu + 1 >= dim<U>(g) || pv + 1 >= dim<V>(g)) return hold;
+    const double ev = E[V * g.nn + id + stride<U>(g)] - E[V * g.nn + id];
+    const double eu = E[U * g.nn + id + stride<V>(g)] - E[U * g.nn + id];
+    double t1 = ev * double(g.ax[U].inv_d[pu]), t2 = eu * double(g.ax[V].inv_d[pv]);
+    if (g.pml) {
+        const int su = g.ax[U].slot_h[pu], sv = g.ax[V].slot_h[pv];
+        if (su >= 0) {
+            double &r = psi[psi_at<U>(g, 1, C, i, j, k, uint32_t(su))];
+            r = double(g.ax[U].bh[pu]) * r + double(g.ax[U].ch[pu]) * ev * double(g.ax[U].raw_d[pu]);
+            t1 += r;
+        }
+        if (sv >= 0) {
+            double &r = psi[psi_at<V>(g, 1, C, i, j, k, uint32_t(sv))];
+            r = double(g.ax[V].bh[pv]) * r + double(g.ax[V].ch[pv]) * eu * double(g.ax[V].raw_d[pv]);
+            t2 += r;
+        }
+    }
+    return hold - G.k_mu * (t1 - t2);
+}
+
+__global__ __launch_bounds__(256) void k_h64(Geo64 G, const double *__restrict__ E, double *__restrict__ H,
+                                             double *__restrict__ psi) {
+    const Geo &g = G.g;
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= g.n2 || j >= g.n1) return;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const double hx = h64<0>(G, E, H, psi, i, j, k), hy = h64<1>(G, E, H, psi, i, j, k), hz = h64<2>(G, E, H, psi, i, j, k);
+    H[id] = hx, H[g.nn + id] = hy, H[2 * g.nn + id] = hz;
+}
+
+template <int C>
+__device__ __forceinline__ double e64(const Geo64 &G, double eold, double hv, double hu, uint32_t i, uint32_t j,
+                                      uint32_t k, uint64_t id, double *__restrict__ psi) {
+    const Geo &g = G.g;
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const double2 cf = G.sets[G.idx[C * g.nn + id]];
+    if (cf.y == 0.0) return eold;
+    const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+    double t1 = hv * double(g.ax[U].inv_dd[pu]), t2 = hu * double(g.ax[V].inv_dd[pv]);
+    if (g.pml) {
+        const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
+        if (su >= 0) {
+            double &r = psi[psi_at<U>(g, 0, C, i, j, k, uint32_t(su))];
+            r = double(g.ax[U].be[pu]) * r + double(g.ax[U].ce[pu]) * hv * double(g.ax[U].raw_dd[pu]);
+            t1 += r;
+        }
+        if (sv >= 0) {
+            double &r = psi[psi_at<V>(g, 0, C, i, j, k, uint32_t(sv))];
+            r = double(g.ax[V].be[pv]) * r + double(g.ax[V].ce[pv]) * hu * double(g.ax[V].raw_dd[pv]);
+            t2 += r;
+        }
+    }
+    return cf.x * eold + cf.y * (t1 - t2);
+}
+
+__global__ __launch_bounds__(256) void k_e64(Geo64 G, double *__restrict__ E, const double *__restrict__ H,
+                                             double *__restrict__ psi) {
+    const Geo &g = G.g;
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= g.n2 || j >= g.n1) return;
+    const uint64_t nn = g.nn, id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const uint64_t im = i ? id - stride<0>(g) : id, jm = j ? id - stride<1>(g) : id, km = k ? id - 1 : id;
+    const double hx = H[id], hy = H[nn + id], hz = H[2 * nn + id];
+    const double ex = e64<0>(G, E[id], hz - H[2 * nn + jm], hy - H[nn + km], i, j, k, id, psi);
+    const double ey = e64<1>(G, E[nn + id], hx - H[km], hz - H[2 * nn + im], i, j, k, id, psi);
+    const double ez = e64<2>(G, E[2 * nn + id], hy - H[nn + im], hx - H[jm], i, j, k, id, psi);
+    E[id] = ex, E[nn + id] = ey, E[2 * nn + id] = ez;
+}
+
+// sources of the driven port (cpu.rs:291-297) with the pulse (sim.rs:646-657) in f64, then the
+// observations of this step
+__global__ void k_fix64(uint64_t nn, const fg_source *__restrict__ src, uint32_t n_src, double *__restrict__ E,
+                        uint64_t n, double dt, double t0, double tau, double w0, const uint64_t *__restrict__ obs,
+                        uint32_t n_obs, double *__restrict__ series) {
+    const uint32_t q = threadIdx.x;
+    const double t = (double(n) + 0.5) * dt, x = (t - t0) / tau;
+    const double pl = exp(-x * x) * cos(w0 * (t - t0));
+    for (uint32_t s = q; s < n_src; s += blockDim.x) E[src[s].comp * nn + src[s].id] -= double(src[s].src) * pl;
+    __syncthreads();
+    for (uint32_t o = q; o < n_obs; o += blockDim.x) series[n * n_obs + o] = E[obs[o]];
+}
+
+}  // namespace fg
+
+// ---- host ------------------------------------------------------------------------------------
+
+struct fg_adjoint {
+    fg_device *dev = nullptr;
+    fg_sim_desc desc{};
+    std::vector<float> eps;  // owned copy of the cell eps_r the desc points at
+    std::vector<fg_point_probe> obs;
+    fg_design design{};
+    uint32_t driven = 0;
+    fg_pulse pulse{};
+    uint64_t box_cells = 0, steps = 0;
+    uint32_t every = 100;
+    fg_sim *sim = nullptr;
+    fg_problem problem{};
+    std::unique_ptr<fg::Session> s;
+    std::vector<fg::DesignEdge> edges;
+    bool built = false, have_forward = false;
+    fg::DevBuf<float> snaps, lamE, lamH, lamP[2];
+    fg::DevBuf<uint64_t> d_edges, d_obs;
+    fg::DevBuf<float2> tape;
+    fg::DevBuf<double> gca, gcb, dseries;
+    uint64_t n_snaps = 0, snap_len = 0;
+    double ms[4] = {0, 0, 0, 0};
+
+    ~fg_adjoint() {
+        s.reset();
+        if (sim) fg_sim_free(sim);
+    }
+};
+
+namespace fg {
+
+static uint64_t box_index(const fg_design &d, uint64_t i, uint64_t j, uint64_t k) {
+    return ((i - d.lo[0]) * (d.hi[1] - d.lo[1]) + (j - d.lo[1])) * (d.hi[2] - d.lo[2]) + (k - d.lo[2]);
+}
+
+// (Re)build the problem from the desc with the current eps copy, and its fused session.
+static void rebuild(fg_adjoint &A) {
+    A.s.reset();
+    if (A.sim) fg_sim_free(A.sim);
+    A.sim = nullptr;
+    if (fg_sim_new(&A.desc, &A.sim) != FG_OK) throw HipError{g_error, FG_INVALID};
+    uint64_t counts[4];
+    fg_sim_counts(A.sim, counts);
+    if (counts[0] || counts[1] || counts[2] || counts[3])
+        throw HipError{"the adjoint covers curl updates, CPML and resistive ports; this problem has Debye media, "
+                       "inductors, resistive sheets or thin wires",
+                       FG_UNSUPPORTED};
+    if (fg_sim_pack(A.sim, 0, &A.problem) != FG_OK) throw HipError{g_error, FG_INVALID};
+    fg_extras x{};
+    x.max_steps = A.steps;
+    x.decay_db = INFINITY;
+    x.engine = FG_ENGINE_FUSED;
+    x.plane_k = -1;
+    x.n_probes = uint32_t(A.obs.size());
+    x.probes = A.obs.data();
+    A.s.reset(session_new(A.dev, &A.problem, A.driven, &A.pulse, &x));
+    if (A.edges.empty()) {
+        const std::string err = design_map(*A.sim, A.desc, A.design.lo, A.design.hi, A.edges);
+        if (!err.empty()) throw HipError{err, FG_INVALID};
+    }
+    A.built = true;
+}
+
+static void set_box(fg_adjoint &A, const float *eps_box) {
+    const uint64_t n1 = A.desc.len[1] - 1, n2 = A.desc.len[2] - 1;
+    const fg_design &d = A.design;
+    bool changed = false;
+    for (uint64_t i = d.lo[0]; i < d.hi[0]; i++)
+        for (uint64_t j = d.lo[1]; j < d.hi[1]; j++)
+            for (uint64_t k = d.lo[2]; k < d.hi[2]; k++) {
+                float &v = A.eps[(i * n1 + j) * n2 + k];
+                const float w = eps_box[box_index(d, i, j, k)];
+                changed |= v != w;
+                v = w;
+            }
+    if (changed) A.built = false;
+}
+
+static Geo geo(const fg_adjoint &A) { return A.s->geo; }
+
+static void forward(fg_adjoint &A, float *series) {
+    if (!A.built) rebuild(A);
+    Session &s = *A.s;
+    hipStream_t st = s.stream;
+    reset(s);
+    const uint64_t psi_n = s.psi[0].n, nn = s.nn;
+    A.snap_len = 6 * nn + psi_n;
+    A.n_snaps = (A.steps + A.every - 1) / A.every;  // segment k starts at k*every; k = 0 is all zeros
+    if (A.snaps.n < (A.n_snaps ? A.n_snaps - 1 : 0) * A.snap_len) {
+        size_t free_b = 0, total_b = 0;
+        FG_CHECK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t need = (A.n_snaps - 1) * A.snap_len * 4;
+        if (need > free_b)
+            throw HipError{"the field snapshots (" + std::to_string(need >> 20) + " MiB) do not fit in this GPU", FG_TOO_LARGE};
+        A.snaps.alloc(std::max<uint64_t>((A.n_snaps - 1) * A.snap_len, 1));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t k = 0; k < A.n_snaps; k++) {
+        if (k) {  // snapshot at the start of segment k
+            float *d = A.snaps.p + (k - 1) * A.snap_len;
+            FG_CHECK(hipMemcpyAsync(d, s.E[s.cur].p, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            FG_CHECK(hipMemcpyAsync(d + 3 * nn, s.H[s.cur].p, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            if (psi_n) FG_CHECK(hipMemcpyAsync(d + 6 * nn, s.psi[s.cur].p, psi_n * 4, hipMemcpyDeviceToDevice, st));
+        }
+        enqueue_steps(s, std::min<uint64_t>(A.every, A.steps - k * A.every));
+    }
+    FG_CHECK(hipStreamSynchronize(st));
+    A.ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (series && A.steps && !A.obs.empty())
+        FG_CHECK(hipMemcpy(series, s.probes.p, A.steps * A.obs.size() * 4, hipMemcpyDeviceToHost));
+    A.have_forward = true;
+}
+
+static void backward(fg_adjoint &A, const double *dseries, double *grad) {
+    if (!A.have_forward) throw HipError{"fg_adjoint_backward needs a forward run first", FG_INVALID};
+    Session &s = *A.s;
+    hipStream_t st = s.stream;
+    const uint64_t nn = s.nn, psi_n = s.psi[0].n, ne = A.edges.size(), n_obs = A.obs.size();
+    const Geo g = geo(A);
+    if (A.lamE.n != 3 * nn) {
+        A.lamE.alloc(3 * nn);
+        A.lamH.alloc(3 * nn);
+        A.lamP[0].alloc(std::max<uint64_t>(psi_n, 1));
+        A.lamP[1].alloc(std::max<uint64_t>(psi_n, 1));
+        std::vector<uint64_t> f(ne), o(n_obs);
+        for (uint64_t e = 0; e < ne; e++) f[e] = A.edges[e].f;
+        for (uint64_t q = 0; q < n_obs; q++)
+            o[q] = A.obs[q].field * nn + (uint64_t(A.obs[q].i) * s.n[1] + A.obs[q].j) * s.n[2] + A.obs[q].k;
+        A.d_edges.upload(f.data(), f.size(), st);
+        A.d_obs.upload(o.data(), o.size(), st);
+        A.tape.alloc(std::max<uint64_t>(A.every * ne, 1));
+        A.gca.alloc(std::max<uint64_t>(ne, 1));
+        A.gcb.alloc(std::max<uint64_t>(ne, 1));
+    }
+    A.lamE.zero(st), A.lamH.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
+    A.gca.zero(st), A.gcb.zero(st);
+    A.dseries.upload(dseries, A.steps * n_obs, st);
+    std::vector<hipEvent_t> ev(3 * A.n_snaps);
+    for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
+    const dim3 block(64, 4), grid((g.n2 + 63) / 64, (g.n1 + 3) / 4, g.n0);
+    const uint32_t eb = uint32_t((ne + 255) / 256);
+    int pin = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t k = A.n_snaps; k-- > 0;) {
+        const uint64_t start = k * A.every, end = std::min(start + A.every, A.steps);
+        FG_CHECK(hipEventRecord(ev[3 * k], st));
+        // restore the segment's starting fields into copy 0
+        if (k == 0) {
+            s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
+        } else {
+            const float *d = A.snaps.p + (k - 1) * A.snap_len;
+            FG_CHECK(hipMemcpyAsync(s.E[0].p, d, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            FG_CHECK(hipMemcpyAsync(s.H[0].p, d + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, d + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
+        }
+        s.cur = 0;
+        s.steps = start;
+        for (uint64_t n = start; n < end; n++) {
+            const int prev = s.cur;
+            enqueue_steps(s, 1);
+            hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
+                               s.H[s.cur].p, A.tape.p + (n - start) * ne);
+        }
+        FG_CHECK(hipEventRecord(ev[3 * k + 1], st));
+        for (uint64_t n = end; n-- > start;) {
+            if (n_obs)
+                hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64 * ((n_obs + 63) / 64)), 0, st, A.lamE.p, A.d_obs.p,
+                                   uint32_t(n_obs), A.dseries.p + n * n_obs);
+            if (ne)
+                hipLaunchKernelGGL(k_adj_grad, dim3(eb), dim3(256), 0, st, A.d_edges.p, uint32_t(ne), A.lamE.p,
+                                   A.tape.p + (n - start) * ne, A.gca.p, A.gcb.p);
+            hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamE.p, A.lamH.p, A.lamP[pin].p);
+            hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamH.p, A.lamP[pin].p, A.lamP[pin ^ 1].p);
+            pin ^= 1;
+        }
+        FG_CHECK(hipEventRecord(ev[3 * k + 2], st));
+        FG_CHECK(hipGetLastError());
+    }
+    FG_CHECK(hipStreamSynchronize(st));
+    A.ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    A.ms[2] = A.ms[3] = 0.0;
+    for (uint64_t k = 0; k < A.n_snaps; k++) {
+        float a = 0.f, b = 0.f;
+        FG_CHECK(hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]));
+        FG_CHECK(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
+        A.ms[2] += a, A.ms[3] += b;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    // chain rule per design edge: ca = (1-x)/(1+x), cb = dt/(eps (1+x)), x = sig dt / (2 eps)
+    // (sim.rs:431-441), eps = EPS0 (fixed + sum w eps_r)
+    std::vector<double> ga(ne), gb(ne);
+    if (ne) {
+        FG_CHECK(hipMemcpy(ga.data(), A.gca.p, ne * 8, hipMemcpyDeviceToHost));
+        FG_CHECK(hipMemcpy(gb.data(), A.gcb.p, ne * 8, hipMemcpyDeviceToHost));
+    }
+    std::fill(grad, grad + A.box_cells, 0.0);
+    const double dt = fg_sim_dt(A.sim), eps0 = 8.8541878128e-12;
+    const uint64_t n1 = A.desc.len[1] - 1, n2 = A.desc.len[2] - 1;
+    const fg_design &d = A.design;
+    for (uint64_t e = 0; e < ne; e++) {
+        const DesignEdge &de = A.edges[e];
+        double er = de.fixed;
+        for (int q = 0; q < 4; q++)
+            if (de.cell[q] != UINT32_MAX) {
+                const uint64_t bz = d.hi[2] - d.lo[2], by = d.hi[1] - d.lo[1];
+                const uint64_t ci = de.cell[q] / (by * bz) + d.lo[0], cj = (de.cell[q] / bz) % by + d.lo[1],
+                               ck = de.cell[q] % bz + d.lo[2];
+                er += de.w[q] * double(A.eps[(ci * n1 + cj) * n2 + ck]);
+            }
+        const double eps = eps0 * er, x = de.sig * dt / (2.0 * eps);
+        const double dca = 2.0 * x / (eps * (1.0 + x) * (1.0 + x)), dcb = -dt / (eps * eps * (1.0 + x) * (1.0 + x));
+        const double gedge = ga[e] * dca + gb[e] * dcb;
+        for (int q = 0; q < 4; q++)
+            if (de.cell[q] != UINT32_MAX) grad[de.cell[q]] += gedge * eps0 * de.w[q];
+    }
+}
+
+static void forward64(fg_adjoint &A, const double *eps_box, double *series) {
+    if (!A.built) rebuild(A);
+    Session &s = *A.s;
+    hipStream_t st = s.stream;
+    const uint64_t nn = s.nn, psi_n = s.psi[0].n, n_obs = A.obs.size();
+    const fg_problem &P = A.problem;
+    // coefficient sets in f64: the problem's sets, then one pair per design edge from eps_box
+    std::vector<double2> sets(P.n_sets);
+    for (uint32_t q = 0; q < P.n_sets; q++) sets[q] = make_double2(P.coef_sets[2 * q], P.coef_sets[2 * q + 1]);
+    std::vector<uint32_t> idx(3 * nn);
+    const uint16_t *narrow = reinterpret_cast<const uint16_t *>(P.coef_index);
+    for (uint64_t f = 0; f < 3 * nn; f++) idx[f] = P.coef_wide ? P.coef_index[f] : narrow[f];
+    const double dt = fg_sim_dt(A.sim), eps0 = 8.8541878128e-12;
+    const uint64_t n1 = A.desc.len[1] - 1, n2 = A.desc.len[2] - 1;
+    const fg_design &d = A.design;
+    const uint64_t bz = d.hi[2] - d.lo[2], by = d.hi[1] - d.lo[1];
+    for (const DesignEdge &de : A.edges) {
+        double er = de.fixed;
+        for (int q = 0; q < 4; q++)
+            if (de.cell[q] != UINT32_MAX) {
+                const uint64_t ci = de.cell[q] / (by * bz) + d.lo[0], cj = (de.cell[q] / bz) % by + d.lo[1],
+                               ck = de.cell[q] % bz + d.lo[2];
+                er += de.w[q] * (eps_box ? eps_box[de.cell[q]] : double(A.eps[(ci * n1 + cj) * n2 + ck]));
+            }
+        const double eps = eps0 * er, x = de.sig * dt / (2.0 * eps);
+        idx[de.f] = uint32_t(sets.size());
+        sets.push_back(make_double2((1.0 - x) / (1.0 + x), dt / eps / (1.0 + x)));
+    }
+    DevBuf<double2> dsets;
+    DevBuf<uint32_t> didx;
+    DevBuf<double> E, H, psi, out;
+    DevBuf<uint64_t> dobs;
+    DevBuf<fg_source> src;
+    dsets.upload(sets.data(), sets.size(), st);
+    didx.upload(idx.data(), idx.size(), st);
+    E.alloc(3 * nn), H.alloc(3 * nn), psi.alloc(std::max<uint64_t>(psi_n, 1)), out.alloc(std::max<uint64_t>(A.steps * n_obs, 1));
+    E.zero(st), H.zero(st), psi.zero(st);
+    std::vector<uint64_t> o(n_obs);
+    for (uint64_t q = 0; q < n_obs; q++)
+        o[q] = A.obs[q].field * nn + (uint64_t(A.obs[q].i) * s.n[1] + A.obs[q].j) * s.n[2] + A.obs[q].k;
+    dobs.upload(o.data(), o.size(), st);
+    uint32_t n_src = 0;
+    if (P.n_ports) {
+        const fg_port &dp = P.ports[A.driven];
+        n_src = dp.n_sources;
+        src.upload(P.sources + dp.first_source, n_src, st);
+    }
+    Geo64 G{s.geo, didx.p, dsets.p, dt / 1.25663706212e-6};
+    const double tau = std::sqrt(std::log(10.0)) / (3.14159265358979323846 * A.pulse.fc), t0 = 4.0 * tau;
+    const double w0 = 2.0 * 3.14159265358979323846 * A.pulse.f0;
+    const dim3 block(64, 4), grid((s.n[2] + 63) / 64, (s.n[1] + 3) / 4, s.n[0]);
+    for (uint64_t n = 0; n < A.steps; n++) {
+        hipLaunchKernelGGL(k_h64, grid, block, 0, st, G, E.p, H.p, psi.p);
+        hipLaunchKernelGGL(k_e64, grid, block, 0, st, G, E.p, H.p, psi.p);
+        hipLaunchKernelGGL(k_fix64, dim3(1), dim3(64), 0, st, nn, src.p, n_src, E.p, n, dt, t0, tau, w0, dobs.p,
+                           uint32_t(n_obs), out.p);
+    }
+    FG_CHECK(hipGetLastError());
+    if (series && A.steps * n_obs)
+        FG_CHECK(hipMemcpyAsync(series, out.p, A.steps * n_obs * 8, hipMemcpyDeviceToHost, st));
+    FG_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace fg
+
+extern "C" {
+
+int fg_adjoint_new(fg_device *dev, const fg_sim_desc *desc, const fg_design *design, uint32_t driven,
+                   const fg_pulse *pulse, fg_adjoint **out) {
+    using namespace fg;
+    return guarded([&] {
+        if (!dev || !desc || !design || !pulse || !out) throw HipError{"fg_adjoint_new: NULL argument", FG_INVALID};
+        auto A = std::make_unique<fg_adjoint>();
+        A->dev = dev;
+        A->desc = *desc;
+        for (int a = 0; a < 3; a++)
+            if (!desc->line[a] || desc->len[a] < 2) throw HipError{"a grid line needs at least two nodes", FG_INVALID};
+        const uint64_t cells = uint64_t(desc->len[0] - 1) * (desc->len[1] - 1) * (desc->len[2] - 1);
+        A->eps.assign(cells, 1.0f);
+        if (desc->eps) std::memcpy(A->eps.data(), desc->eps, cells * 4);
+        A->desc.eps = A->eps.data();
+        A->design = *design;
+        A->obs.assign(design->obs, design->obs + design->n_obs);
+        for (const fg_point_probe &o : A->obs)
+            if (o.field > 2 || o.i >= desc->len[0] || o.j >= desc->len[1] || o.k >= desc->len[2])
+                throw HipError{"an observation is not an E component inside the grid", FG_INVALID};
+        for (int a = 0; a < 3; a++)
+            if (design->lo[a] >= design->hi[a] || design->hi[a] > desc->len[a] - 1)
+                throw HipError{"the design box lies outside the grid", FG_INVALID};
+        A->design.obs = A->obs.data();
+        A->box_cells = uint64_t(design->hi[0] - design->lo[0]) * (design->hi[1] - design->lo[1]) * (design->hi[2] - design->lo[2]);
+        A->steps = design->steps;
+        A->every = design->every ? design->every : 100;
+        A->driven = driven;
+        A->pulse = *pulse;
+        FG_CHECK(hipSetDevice(dev->ordinal));
+        rebuild(*A);
+        *out = A.release();
+        return int(FG_OK);
+    });
+}
+
+int fg_adjoint_forward(fg_adjoint *adj, const float *eps_box, float *series) {
+    using namespace fg;
+    return guarded([&] {
+        if (!adj) throw HipError{"fg_adjoint_forward: NULL handle", FG_INVALID};
+        FG_CHECK(hipSetDevice(adj->dev->ordinal));
+        if (eps_box) set_box(*adj, eps_box);
+        forward(*adj, series);
+        return int(FG_OK);
+    });
+}
+
+int fg_adjoint_backward(fg_adjoint *adj, const double *dseries, double *grad) {
+    using namespace fg;
+    return guarded([&] {
+        if (!adj || !dseries || !grad) throw HipError{"fg_adjoint_backward: NULL argument", FG_INVALID};
+        FG_CHECK(hipSetDevice(adj->dev->ordinal));
+        backward(*adj, dseries, grad);
+        return int(FG_OK);
+    });
+}
+
+int fg_adjoint_forward64(fg_adjoint *adj, const double *eps_box, double *series) {
+    using namespace fg;
+    return guarded([&] {
+        if (!adj) throw HipError{"fg_adjoint_forward64: NULL handle", FG_INVALID};
+        FG_CHECK(hipSetDevice(adj->dev->ordinal));
+        forward64(*adj, eps_box, series);
+        return int(FG_OK);
+    });
+}
+
+int fg_adjoint_timing(const fg_adjoint *adj, double ms[4], uint64_t *snapshots) {
+    if (!adj) return fg::set_error(FG_INVALID, "fg_adjoint_timing: NULL handle");
+    if (ms)
+        for (int q = 0; q < 4; q++) ms[q] = adj->ms[q];
+    if (snapshots) *snapshots = adj->n_snaps ? adj->n_snaps - 1 : 0;
+    return FG_OK;
+}
+
+void fg_adjoint_free(fg_adjoint *adj) { delete adj; }
+
+}  // extern "C"

@@ -421,6 +421,12 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     if (const char *v = getenv("FIELDGRID_XCDS")) s.xcds = std::max(1u, uint32_t(atoi(v)));
     if (s.tj != 8 && s.tj != 14) s.tj = 14;
     s.seg = std::min<uint32_t>(s.seg, SEG_MAX);
+    // a grid too small to give ~6 blocks per CU at `seg` planes per block marches shorter
+    // segments instead (the results do not depend on the tiling)
+    if (!getenv("FIELDGRID_SEG"))
+        while (s.seg > 4 && uint64_t((P->n[2] + OWN_K - 1) / OWN_K) * ((P->n[1] + s.tj - 1) / s.tj) *
+                                    ((P->n[0] + s.seg - 1) / s.seg) < 6ull * uint64_t(std::max(dev->cus, 1)))
+            s.seg /= 2;
     s.pml = P->pml;
     s.nmax = nmax;
     s.slots_host = axs;
@@ -532,7 +538,7 @@ static void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     FG_CHECK(hipGetLastError());
 }
 
-static void enqueue_steps(Session &s, uint64_t count) {
+void enqueue_steps(Session &s, uint64_t count) {
     if (s.timing) {
         collect_timing(s);
         // timed steps go in batches so the event pool stays bounded

@@ -152,6 +152,9 @@ void xcd_order(std::vector<T> &v, size_t xcds, const T &pad) {
     v.swap(out);
 }
 
+// Enqueue `count` further steps on the session's stream (asynchronous).
+void enqueue_steps(Session &s, uint64_t count);
+
 // Sum of E^2 + (376.73 H)^2 over the latest fields (cpu.rs:417-421); synchronizes the stream.
 double energy(Session &s);
 // Copy the run's outputs into a caller-allocated record (run.rs:746-757).

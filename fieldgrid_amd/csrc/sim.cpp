@@ -619,6 +619,78 @@ static void coef_sets(fg_sim &s, uint64_t narrow) {
     });
 }
 
+static void design_map_impl(const fg_sim &s, const fg_sim_desc &d, const uint32_t lo[3], const uint32_t hi[3],
+                            std::vector<DesignEdge> &out) {
+    const uint64_t *n = s.n;
+    const uint64_t nn = s.nn, cells[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
+    for (int a = 0; a < 3; a++)
+        if (lo[a] >= hi[a] || hi[a] > cells[a]) throw BuildError{"the design box lies outside the grid"};
+    const uint64_t box[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    std::unordered_set<uint64_t> excluded, debye, seen;
+    auto flat = [&](const fg_edge &e) { return uint64_t(e.comp) * nn + node(n, e.at[0], e.at[1], e.at[2]); };
+    for (uint32_t l = 0; l < d.n_lumped; l++)
+        for (uint32_t q = 0; q < d.lumped[l].n_edges; q++) excluded.insert(flat(d.lumped[l].edges[q]));
+    for (uint32_t p = 0; p < d.n_ports; p++) {
+        uint64_t o = 0;
+        for (uint32_t c = 0; c < d.ports[p].n_columns; c++)
+            for (uint32_t q = 0; q < d.ports[p].column_len[c]; q++, o++) excluded.insert(flat(d.ports[p].edges[o]));
+    }
+    for (uint32_t r = 0; r < d.n_resistive; r++) excluded.insert(uint64_t(d.resistive[r].comp) * nn + d.resistive[r].id);
+    for (const fg_debye_edge &e : s.debye) debye.insert(e.f);
+    out.clear();
+    static const int offs[4][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}};
+    for (uint64_t ci = lo[0]; ci < hi[0]; ci++)
+        for (uint64_t cj = lo[1]; cj < hi[1]; cj++)
+            for (uint64_t ck = lo[2]; ck < hi[2]; ck++)
+                for (uint32_t c = 0; c < 3; c++) {
+                    const uint32_t u = (c + 1) % 3, v = (c + 2) % 3;
+                    for (uint32_t du = 0; du < 2; du++)
+                        for (uint32_t dv = 0; dv < 2; dv++) {
+                            uint64_t p[3] = {ci, cj, ck};
+                            p[u] += du;
+                            p[v] += dv;
+                            const uint64_t f = uint64_t(c) * nn + node(n, p[0], p[1], p[2]);
+                            if (!seen.insert(f).second) continue;
+                            if (s.cb[f] == 0.f || debye.count(f)) continue;  // eps does not enter
+                            if (excluded.count(f)) throw BuildError{"the design box touches a port, lumped or resistive edge"};
+                            if (s.ax[u].slot_e[p[u]] >= 0 || s.ax[v].slot_e[p[v]] >= 0)
+                                throw BuildError{"the design box reaches into the CPML"};
+                            DesignEdge e{f, 0.0, 0.0, {}, {}};
+                            double sum_s = 0.0, w_sum = 0.0;
+                            for (int q = 0; q < 4; q++) {
+                                int64_t cell[3] = {int64_t(p[0]), int64_t(p[1]), int64_t(p[2])};
+                                cell[u] += offs[q][0];
+                                cell[v] += offs[q][1];
+                                const double w = s.ax[u].d[size_t(cell[u])] * s.ax[v].d[size_t(cell[v])];
+                                const uint64_t cid = (uint64_t(cell[0]) * cells[1] + uint64_t(cell[1])) * cells[2] + uint64_t(cell[2]);
+                                bool in = true;
+                                for (int a = 0; a < 3; a++) in = in && uint64_t(cell[a]) >= lo[a] && uint64_t(cell[a]) < hi[a];
+                                e.cell[q] = in ? uint32_t(((uint64_t(cell[0]) - lo[0]) * box[1] + (uint64_t(cell[1]) - lo[1])) * box[2] +
+                                                          (uint64_t(cell[2]) - lo[2]))
+                                               : UINT32_MAX;
+                                e.w[q] = w;
+                                if (!in) e.fixed += (d.eps ? double(d.eps[cid]) : 1.0) * w;
+                                sum_s += (d.sigma ? double(d.sigma[cid]) : 0.0) * w;
+                                w_sum += w;
+                            }
+                            for (int q = 0; q < 4; q++) e.w[q] /= w_sum;
+                            e.fixed /= w_sum;
+                            e.sig = sum_s / w_sum;
+                            out.push_back(e);
+                        }
+                }
+}
+
+std::string design_map(const fg_sim &s, const fg_sim_desc &d, const uint32_t lo[3], const uint32_t hi[3],
+                       std::vector<DesignEdge> &out) {
+    try {
+        design_map_impl(s, d, lo, hi, out);
+        return {};
+    } catch (const BuildError &e) {
+        return e.msg;
+    }
+}
+
 }  // namespace fg
 
 // ---- C-ABI -------------------------------------------------------------------------------

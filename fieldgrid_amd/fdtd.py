@@ -6,7 +6,8 @@ Same names, argument meanings and error behaviour as the reference's Rust API:
   pulse_shape, idx, unidx, psi_len, and the constants C0, EPS0, MU0, ETA0.
 `Sim.try_new` runs the C++ builder in libfieldgrid.so; `run` runs the HIP engine on an
 MI355X through the C-ABI of include/fieldgrid.h. Beyond the reference: `run_batch` (many runs
-in shared launches, include/fieldgrid_batch.h) with `BatchRun`, `Tune` and `port_spectra`. Errors raise FieldgridError carrying the
+in shared launches, include/fieldgrid_batch.h) with `BatchRun`, `Tune` and `port_spectra`;
+`Design` (the discrete adjoint, include/fieldgrid_adjoint.h). Errors raise FieldgridError carrying the
 reference's message ("cancelled", "the FDTD run went unstable", ...).
 
 Two differences of form, not of meaning:
@@ -638,6 +639,71 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
         v, i = (a.reshape(-1, nf, 2) for a in sb)
         spectra.append((v[..., 0] + 1j * v[..., 1], i[..., 0] + 1j * i[..., 1]))
     return records, spectra
+
+
+class Design:
+    """d(objective)/d(eps_r) over a box of cells by the discrete adjoint (fg_adjoint_*; an
+    extension without counterpart in the reference). The contract of a torch.autograd.Function:
+    forward(eps_box) -> series of the observed E components after every step, shape
+    (steps, n_obs); backward(dseries) -> dJ/d eps_r for the box, given dJ/d series.
+    `obs` = [(field 0..2, i, j, k)]; the box [lo, hi) is in cells."""
+
+    def __init__(self, gpu: Gpu, grid, mats, pec, lumped, ports, resistive, media, lo, hi, obs, driven,
+                 pulse: Pulse, steps: int, every: int = 100):
+        if gpu is None:
+            raise ValueError("Design needs a Gpu")
+        self.desc = SimDesc(grid, mats, pec, list(lumped), list(ports), list(resistive), media or Media())
+        self.lo, self.hi = tuple(lo), tuple(hi)
+        self.shape = tuple(h - l for l, h in zip(lo, hi))
+        self.steps, self.n_obs = steps, len(obs)
+        self._obs = (abi.fg_point_probe * max(len(obs), 1))(*[abi.fg_point_probe(*o) for o in obs])
+        d = abi.fg_design()
+        d.lo[:], d.hi[:] = list(lo), list(hi)
+        d.n_obs, d.every, d.obs, d.steps = len(obs), every, self._obs, steps
+        self._design = d
+        h = C.c_void_p()
+        abi.check(abi.lib().fg_adjoint_new(gpu._h, C.byref(self.desc.desc), C.byref(d), driven,
+                                           C.byref(abi.fg_pulse(pulse.f0, pulse.fc)), C.byref(h)))
+        self._h = h
+        self.dt = None
+
+    def forward(self, eps_box=None) -> np.ndarray:
+        series = np.zeros(max(self.steps * self.n_obs, 1), np.float32)
+        e = None if eps_box is None else np.ascontiguousarray(eps_box, np.float32).reshape(self.shape)
+        abi.check(abi.lib().fg_adjoint_forward(self._h, None if e is None else e.ctypes.data_as(C.POINTER(C.c_float)),
+                                               series.ctypes.data_as(C.POINTER(C.c_float))))
+        return series[: self.steps * self.n_obs].reshape(self.steps, self.n_obs)
+
+    def backward(self, dseries) -> np.ndarray:
+        ds = np.ascontiguousarray(dseries, np.float64).reshape(-1)
+        if ds.size != self.steps * self.n_obs:
+            raise ValueError("dseries must hold steps x n_obs values")
+        grad = np.zeros(self.shape)
+        abi.check(abi.lib().fg_adjoint_backward(self._h, ds.ctypes.data_as(C.POINTER(C.c_double)),
+                                                grad.ctypes.data_as(C.POINTER(C.c_double))))
+        return grad
+
+    def forward64(self, eps_box=None) -> np.ndarray:
+        series = np.zeros(max(self.steps * self.n_obs, 1))
+        e = None if eps_box is None else np.ascontiguousarray(eps_box, np.float64).reshape(self.shape)
+        abi.check(abi.lib().fg_adjoint_forward64(self._h, None if e is None else e.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 series.ctypes.data_as(C.POINTER(C.c_double))))
+        return series[: self.steps * self.n_obs].reshape(self.steps, self.n_obs)
+
+    def timing(self):
+        """(forward ms, backward ms, backward recompute ms, backward adjoint ms), snapshots."""
+        ms = (C.c_double * 4)()
+        snaps = C.c_uint64()
+        abi.check(abi.lib().fg_adjoint_timing(self._h, ms, C.byref(snaps)))
+        return tuple(ms), int(snaps.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            abi.lib().fg_adjoint_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def port_spectra(series: np.ndarray, ports: int, dt: float, freqs: Sequence[float]):

@@ -1,0 +1,68 @@
+/* fieldgrid discrete adjoint: d(objective)/d(eps_r) over a design box of cells.
+ *
+ * No counterpart in the reference, which has no gradient path. This entry point is for callers
+ * that optimise materials: inverse design, and fitting eps to measurements.
+ *
+ * The contract is that of a torch.autograd.Function:
+ *   forward(eps_box) -> series   E components at observation nodes after every step (fg_run's
+ *                                FP32 engine; point probes, include/fieldgrid.h)
+ *   backward(dseries) -> grad    dJ/d eps_r per box cell, given dJ/d series from the caller's
+ *                                objective J(series)
+ * The backward pass is the exact transpose of the discrete FDTD step: the curl updates, the CPML
+ * psi recursions and the resistive ports (their (ca, cb)). The step is linear and time-invariant
+ * in the fields, so the adjoint recursion runs backward in time on its own fields. The forward
+ * fields enter only through E and the curl term on the design edges. The forward pass keeps a
+ * snapshot of the fields every `every` steps. The backward pass recomputes one segment at a time
+ * from its snapshot, taping the design-edge values, and then runs the adjoint over that segment.
+ * Memory is the snapshots plus one segment's tape.
+ *
+ * Supported problems: curl updates, CPML and resistive ports. Debye media, lumped inductors,
+ * resistive sheets and thin wires give FG_UNSUPPORTED. The box must not touch a port, lumped or
+ * resistive edge, nor reach into the CPML (FG_INVALID).
+ *
+ * fg_adjoint_forward64 runs the same problem in f64 with the box's edge coefficients evaluated
+ * in f64 from a given eps_r: a smooth objective for central finite differences.
+ */
+#ifndef FIELDGRID_ADJOINT_H
+#define FIELDGRID_ADJOINT_H
+
+#include "fieldgrid.h"
+#include "fieldgrid_sim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fg_design {
+    uint32_t lo[3], hi[3];      /* design box [lo, hi) in cells per axis; cell (i,j,k) spans
+                                   nodes i..i+1, j..j+1, k..k+1                               */
+    uint32_t n_obs;
+    uint32_t every;             /* steps between field snapshots (0 = 100)                     */
+    const fg_point_probe *obs;  /* observed E components (field 0..2) at nodes               */
+    uint64_t steps;             /* steps of every forward run                                   */
+} fg_design;
+
+typedef struct fg_adjoint fg_adjoint;
+
+/* Build the problem `desc` (as fg_sim_new) with port `driven` excited by `pulse`. `desc` and
+   the arrays it points at must stay valid until fg_adjoint_free; the eps array is copied. */
+int fg_adjoint_new(fg_device *dev, const fg_sim_desc *desc, const fg_design *design, uint32_t driven,
+                   const fg_pulse *pulse, fg_adjoint **out);
+/* Forward run with snapshots. eps_box (k fastest, NULL = the values in desc or the last call)
+   replaces the box's eps_r; series: steps * n_obs floats. */
+int fg_adjoint_forward(fg_adjoint *adj, const float *eps_box, float *series);
+/* Backward pass for the last forward run: dseries (steps * n_obs doubles, dJ/ds) -> grad
+   (box cells, k fastest, dJ/d eps_r). */
+int fg_adjoint_backward(fg_adjoint *adj, const double *dseries, double *grad);
+/* f64 forward run of the same problem with the box's eps_r = eps_box (NULL = current). */
+int fg_adjoint_forward64(fg_adjoint *adj, const double *eps_box, double *series);
+/* Wall times (ms) of the last forward and backward passes; the backward split into the
+   recompute of the segments and the adjoint steps (HIP events); snapshots kept. */
+int fg_adjoint_timing(const fg_adjoint *adj, double ms[4], uint64_t *snapshots);
+void fg_adjoint_free(fg_adjoint *adj);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIELDGRID_ADJOINT_H */

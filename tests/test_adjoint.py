@@ -1,0 +1,135 @@
+"""The discrete adjoint on the GPU (include/fieldgrid_adjoint.h) against the oracle.
+
+* forward: the FP32 series at the observation is bit-identical to the FP32 oracle's point probe;
+* backward: dJ/d eps_r matches the oracle's FP64 discrete adjoint (itself pinned by central
+  differences in tests/test_adjoint_oracle.py) within 1e-3 of the largest component, and
+  central differences of the device's own f64 forward within 1e-3;
+* the snapshot interval changes nothing (the recomputed segments are the same arithmetic).
+The objective is J = |sum_n Ez_n dt e^{-iw(n+1)dt}|^2 at 8 GHz (workloads.dft_objective).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fieldgrid_amd import abi
+from fieldgrid_amd import fdtd as F
+from fieldgrid_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make(gpu, args, every=100):
+    return F.Design(gpu, args["grid"], args["mats"], args["pec"], args["lumped"], args["ports"], args["resistive"],
+                    args["media"], args["lo"], args["hi"], args["obs"], args["driven"], args["pulse"], args["steps"],
+                    every)
+
+
+def box_of(args):
+    n = args["grid"].dims()
+    cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+    sl = tuple(slice(l, h) for l, h in zip(args["lo"], args["hi"]))
+    return args["mats"].eps.reshape(cells)[sl].copy()
+
+
+def test_the_forward_is_the_fp32_path(gpu):
+    args, freq = W.design_small(steps=300)
+    d = make(gpu, args, every=64)
+    s = d.forward()
+    ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+    o32 = O.run_cpu(d.desc, args["driven"], args["pulse"], ex, 32, fields=False)
+    assert np.array_equal(s[:, 0], np.asarray(o32.probes, np.float32))
+    o64 = O.run_cpu(d.desc, args["driven"], args["pulse"], ex, 64, fields=False)
+    s64 = d.forward64()
+    err = np.max(np.abs(s64[:, 0] - o64.probes)) / np.max(np.abs(o64.probes))
+    print(f"\nforward64 vs fp64 oracle: {err:.2e} (f32 axis tables and coefficients outside the box)")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.02], ids=["lossless", "lossy"])
+def test_the_gradient_matches_the_fp64_adjoint(gpu, sigma):
+    args, freq = W.design_small(steps=300, sigma=sigma)
+    d = make(gpu, args, every=64)
+    s = d.forward()
+    J, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq)
+    g = d.backward(dJ)
+    go, so = O.design_gradient(d.desc, args["driven"], args["pulse"], args["steps"], args["obs"], args["lo"],
+                               args["hi"], dJ)
+    rel = np.max(np.abs(g - go)) / np.max(np.abs(go))
+    print(f"\nJ = {J:.4e}; max |grad - fp64 adjoint| / max |fp64 adjoint| = {rel:.2e}")
+    assert rel < 1e-3
+    big = np.abs(go) > 0.1 * np.max(np.abs(go))
+    assert np.max(np.abs(g[big] - go[big]) / np.abs(go[big])) < 1e-3
+
+
+def test_the_gradient_matches_central_differences_of_the_f64_forward(gpu):
+    args, freq = W.design_small(steps=300, sigma=0.02)
+    d = make(gpu, args, every=50)
+    s = d.forward()
+    dt = F.time_step(args["grid"])
+    _, dJ = W.dft_objective(s[:, 0], dt, freq)
+    g = d.backward(dJ)
+    box = box_of(args).astype(np.float64)
+    rng = np.random.default_rng(3)
+    h = 1e-3
+    for _ in range(6):
+        c = tuple(int(rng.integers(0, n)) for n in box.shape)
+        up, dn = box.copy(), box.copy()
+        up[c] += h
+        dn[c] -= h
+        jp = W.dft_objective(d.forward64(up)[:, 0], dt, freq)[0]
+        jm = W.dft_objective(d.forward64(dn)[:, 0], dt, freq)[0]
+        fd = (jp - jm) / (2 * h)
+        print(f"cell {c}: adjoint {g[c]:.6e}  central difference {fd:.6e}")
+        assert abs(g[c] - fd) < 1e-3 * max(abs(fd), 1e-3 * np.max(np.abs(g)))
+
+
+def test_the_snapshot_interval_does_not_change_the_gradient(gpu):
+    args, freq = W.design_small(steps=200)
+    grads = []
+    for every in (7, 100, 200):
+        d = make(gpu, args, every=every)
+        s = d.forward()
+        _, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq)
+        grads.append(d.backward(dJ))
+        _, snaps = d.timing()
+        assert snaps == math.ceil(200 / every) - 1
+        d.close()
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[1], grads[2])
+
+
+def test_a_new_eps_is_used_by_the_next_forward(gpu):
+    args, freq = W.design_small(steps=120)
+    d = make(gpu, args)
+    base = d.forward()
+    box = box_of(args)
+    box[0, 0, 0] += 0.5
+    moved = d.forward(box)
+    assert not np.array_equal(base, moved)
+    args["mats"].eps.reshape(tuple(n - 1 for n in args["grid"].dims()))[
+        tuple(slice(l, h) for l, h in zip(args["lo"], args["hi"]))] = box
+    ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+    o32 = O.run_cpu(F.SimDesc(args["grid"], args["mats"], None, [], args["ports"], [], args["media"]), 0,
+                    args["pulse"], ex, 32, fields=False)
+    assert np.array_equal(moved[:, 0], np.asarray(o32.probes, np.float32))
+
+
+def test_problems_outside_the_adjoints_reach_are_refused(gpu):
+    args, _ = W.design_small(steps=10)
+    near = dict(args, lo=(2, 11, 10))  # reaches into the 5-cell CPML
+    with pytest.raises(abi.FieldgridError) as e:
+        make(gpu, near)
+    assert e.value.status == abi.FG_INVALID and "CPML" in str(e.value)
+    port = dict(args, lo=(6, 12, 10), hi=(9, 15, 14))  # the port column is at (7, 14)
+    with pytest.raises(abi.FieldgridError) as e:
+        make(gpu, port)
+    assert e.value.status == abi.FG_INVALID and "port" in str(e.value)
+    debye = dict(args, media=F.Media(debye=F.Debye([(1e11, 1.0)])))
+    args["mats"].debye[:] = 0.0
+    m = F.Materials.new(args["grid"])
+    m.eps[:] = args["mats"].eps
+    m.debye[:50] = 1.0
+    with pytest.raises(abi.FieldgridError) as e:
+        make(gpu, dict(debye, mats=m))
+    assert e.value.status == abi.FG_UNSUPPORTED
