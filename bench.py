@@ -9,6 +9,12 @@ observers) over the whole grid. The default workload is the single large through
 line with `roofline` (dominant kernel, HIP-event timed) and `cpu_baseline` (the oracle's
 FP32 restatement of the reference CPU path on a bounded sample of the same recipe).
 
+--workload adjoint measures the inverse-design gradient: 192x192x112 nodes, 0.5 mm, CPML 12, a
+64x64x16-cell design box with eps_r ~ U[1,4], J = |DFT of Ez at a far probe at 8 GHz|^2 over 3000
+steps; forward with snapshots every 100 steps, then the discrete adjoint; reports both times and
+the gradient's error against central differences (h = 1e-3) of the f64 forward on --fd-cells
+seeded cells.
+
 --workload sweep measures the batched sweep instead: 96 seeded runs of uneven grids (48-200
 nodes per axis, CPML 8, 1-4 ports, 800-3000 steps each), first one after another through the
 single-run engine, then together through fg_run_batch (autotuning included), and reports
@@ -42,12 +48,6 @@ def build_workload(name):
         extras_kw = {"boxes": True}
         config = {"workload": "boxes + Debye + port + NTFF", "dims": [256, 256, 256], "dx_mm": 0.5, "pml": 12,
                   "boxes": 40, "debye_boxes": 10, "ntff_freqs_ghz": [2, 5, 10]}
-    elif name == "sweep":
-        runs = W.sweep_runs()
-        r0 = min(runs, key=lambda r: np.prod(r.sim.dims()))
-        sim, pulse = r0.sim, r0.pulse
-        sample = f"the smallest run of the sweep ({'x'.join(map(str, sim.dims()))})"
-        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf)
     elif name == "plain":
         sim, probes = W.plain()
         pulse = W.plain_pulse()
@@ -85,6 +85,17 @@ def cpu_baseline(name, budget_s=20.0):
         pulse = W.boxes_pulse()
         sample = "96^3 version of the boxes recipe (CPML 12, NTFF, Debye)"
         extras = lambda k: W.boxes_extras(sim, steps=k)
+    elif name == "adjoint":
+        args, _ = W.design(dims=(96, 96, 112), box=((32, 64), (32, 64), (48, 64)), port=(15, 48), probe=(81, 48))
+        sim = F.Sim.try_new(args["grid"], args["mats"], None, [], args["ports"], [], args["media"])
+        pulse = args["pulse"]
+        sample = "96x96x112 version of the design recipe (CPML 12, port, design box)"
+        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf, probes=args["obs"])
+    elif name == "sweep":
+        r0 = W.sweep_runs(count=1)[0]
+        sim, pulse = r0.sim, r0.pulse
+        sample = f"the first run of the sweep ({'x'.join(map(str, sim.dims()))})"
+        extras = lambda k: F.Extras(max_steps=k, decay_db=math.inf)
     else:
         sim, probes = W.plain()
         pulse = W.plain_pulse()
@@ -158,12 +169,77 @@ def sweep(args):
     return 0 if same else 1
 
 
+def adjoint(args):
+    """Forward + discrete adjoint of the inverse-design problem, with a finite-difference check."""
+    import numpy as np
+    from fieldgrid_amd import fdtd as F
+    from fieldgrid_amd import workloads as W
+    t_build = time.perf_counter()
+    a, freq = W.design()
+    gpu = F.Gpu(0)
+    d = F.Design(gpu, a["grid"], a["mats"], a["pec"], a["lumped"], a["ports"], a["resistive"], a["media"], a["lo"],
+                 a["hi"], a["obs"], a["driven"], a["pulse"], a["steps"], 100)
+    build_s = time.perf_counter() - t_build
+    n = a["grid"].dims()
+    nn = n[0] * n[1] * n[2]
+    dt = F.time_step(a["grid"])
+    d.forward()  # warm-up (kernels loaded, snapshot memory allocated)
+    series = d.forward()
+    J, dJ = W.dft_objective(series[:, 0], dt, freq)
+    grad = d.backward(dJ)
+    d.backward(dJ)  # a second, warm backward pass is the one timed
+    (fwd_ms, bwd_ms, rec_ms, adj_ms), snaps = d.timing()
+    fd = None
+    if args.fd_cells:
+        n_cells = args.fd_cells
+        box = np.array(a["mats"].eps.reshape(tuple(q - 1 for q in n))[tuple(slice(l, h) for l, h in zip(a["lo"], a["hi"]))],
+                       np.float64)
+        rng = np.random.default_rng(5)
+        flat = rng.choice(box.size, size=n_cells, replace=False)
+        h = 1e-3
+        rows = []
+        t0 = time.perf_counter()
+        for q in flat:
+            c = np.unravel_index(int(q), box.shape)
+            up, dn = box.copy(), box.copy()
+            up[c] += h
+            dn[c] -= h
+            jp = W.dft_objective(d.forward64(up)[:, 0], dt, freq)[0]
+            jm = W.dft_objective(d.forward64(dn)[:, 0], dt, freq)[0]
+            rows.append((float(grad[c]), (jp - jm) / (2 * h)))
+        g_, f_ = np.array(rows).T
+        fd = {"cells": n_cells, "h": h, "max_rel_err": float(np.max(np.abs(g_ - f_) / np.abs(f_))),
+              "max_err_over_max_fd": float(np.max(np.abs(g_ - f_)) / np.max(np.abs(f_))),
+              "median_rel_err": float(np.median(np.abs(g_ - f_) / np.abs(f_))),
+              "seconds": round(time.perf_counter() - t0, 1)}
+    d.close()
+    value = nn * a["steps"] / (fwd_ms * 1e-3) / 1e6
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": a["steps"], "warmup": 1, "ms_per_step": round(fwd_ms / a["steps"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded design box)",
+        "config": {"workload": "adjoint inverse-design gradient", "dims": list(n), "dx_mm": 0.5, "pml": 12,
+                   "design_box_cells": [h - l for l, h in zip(a["lo"], a["hi"])], "objective": "|DFT Ez at 8 GHz|^2",
+                   "snapshot_every": 100, "build_s": round(build_s, 2), "gpu": gpu.name},
+        "adjoint": {"forward_ms": round(fwd_ms, 1), "backward_ms": round(bwd_ms, 1),
+                    "backward_over_forward": round(bwd_ms / fwd_ms, 3),
+                    "recompute_ms": round(rec_ms, 1), "adjoint_steps_ms": round(adj_ms, 1), "snapshots": snaps,
+                    "J": J, "grad_max": float(np.max(np.abs(grad))), "finite_differences": fd},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline("adjoint")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain", "sweep"])
+    ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain", "sweep", "adjoint"])
+    ap.add_argument("--fd-cells", type=int, default=32, help="adjoint: design cells checked by central differences")
     ap.add_argument("--engine", default="fused", choices=["fused", "split"])
     ap.add_argument("--timed-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,6 +250,8 @@ def main():
         return 2
     if args.workload == "sweep":
         return sweep(args)
+    if args.workload == "adjoint":
+        return adjoint(args)
     from fieldgrid_amd import abi
     from fieldgrid_amd import fdtd as F
     from fieldgrid_amd import workloads as W

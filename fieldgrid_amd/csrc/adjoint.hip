@@ -7,7 +7,8 @@
 //   lam^n = M^T lam^{n+1} + dJ/dx^n,  dJ/dca_e += lamE_e^{n+1} E_e^n,  dJ/dcb_e += lamE_e^{n+1} curl_e^n.
 // M^T is written in gather form, so each thread owns its outputs:
 //   k_adj_e  transposes the E update: H-bar gathers the "a = cb lamE" terms of the E edges whose
-//            curl reads it, at the node and its +1 neighbours (the shape of an H update);
+//            curl reads it, at the node and its +1 neighbours (the shape of an H update); a is
+//            kept as a field of its own (la), written by k_adj_h, so no coefficient is read here;
 //   k_adj_h  transposes the H update and finishes lamE = ca lamE + (the "b = -k_mu lamH" terms
 //            of the H components whose curl reads it, at the node and its -1 neighbours), and
 //            advances every CPML psi adjoint into the other ping-pong copy.
@@ -22,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -32,20 +34,19 @@
 
 namespace fg {
 
-// hv / hu adjoint terms of E edge C at node (i,j,k): with a = cb lamE, the E update's
-// t1 = hv inv_dd_u + psi_u', t2 = hu inv_dd_v + psi_v', psi' = be psi + ce (h diff) raw_dd gives
+// hv / hu adjoint terms of E edge C at node (i,j,k): with a = cb lamE (kept as its own field
+// `la`, so no coefficient is looked up here), the E update's t1 = hv inv_dd_u + psi_u',
+// t2 = hu inv_dd_v + psi_v', psi' = be psi + ce (h diff) raw_dd gives
 //   hv-bar = a inv_dd_u + ce_u raw_dd_u (psiBar_u + a),  hu-bar = -a inv_dd_v + ce_v raw_dd_v (psiBar_v - a).
-// Zero outside the grid and where cb = 0 (the edge is never updated).
+// Zero outside the grid; an edge with cb = 0 has a = 0 and a psi adjoint that stays 0.
 template <int C>
-__device__ __forceinline__ void e_terms(const Geo &g, const float *__restrict__ lamE, const float *__restrict__ lp,
+__device__ __forceinline__ void e_terms(const Geo &g, const float *__restrict__ la, const float *__restrict__ lp,
                                         uint32_t i, uint32_t j, uint32_t k, float &hv, float &hu) {
     constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
     hv = 0.f, hu = 0.f;
     if (i >= g.n0 || j >= g.n1 || k >= g.n2) return;
     const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
-    const float cb = coef_at(g, C * g.nn + id).y;
-    if (cb == 0.f) return;
-    const float a = cb * lamE[C * g.nn + id];
+    const float a = la[C * g.nn + id];
     const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
     hv = a * g.ax[U].inv_dd[pu];
     hu = -a * g.ax[V].inv_dd[pv];
@@ -79,27 +80,54 @@ __device__ __forceinline__ void h_terms(const Geo &g, const float *__restrict__ 
     }
 }
 
+// Design edges of one step's gradient, looked up per node of the box (nodes lo..hi per axis).
+struct GradArgs {
+    uint32_t lo[3], nb[3];     // first node and node counts of the box's edges
+    const int32_t *map;        // [c][box node] -> design edge, -1 where eps does not enter
+    const float2 *tape;        // this step's (E^n, curl^n) per design edge
+    double *gca, *gcb;         // dJ/dca, dJ/dcb per design edge
+};
+
+// lamE^{n+1} (E^n, curl^n) summed into dJ/dca, dJ/dcb on the design edges at node (i,j,k)
+__device__ __forceinline__ void grad_terms(const Geo &g, const GradArgs &ga, const float *__restrict__ lamE,
+                                           uint32_t i, uint32_t j, uint32_t k, uint64_t id) {
+    const uint32_t bi = i - ga.lo[0], bj = j - ga.lo[1], bk = k - ga.lo[2];
+    if (bi >= ga.nb[0] || bj >= ga.nb[1] || bk >= ga.nb[2]) return;
+    const uint64_t b = (uint64_t(bi) * ga.nb[1] + bj) * ga.nb[2] + bk, nb = uint64_t(ga.nb[0]) * ga.nb[1] * ga.nb[2];
+    for (int c = 0; c < 3; c++) {
+        const int32_t e = ga.map[c * nb + b];
+        if (e < 0) continue;
+        const double l = lamE[c * g.nn + id];
+        const float2 tp = ga.tape[e];
+        ga.gca[e] += l * double(tp.x);
+        ga.gcb[e] += l * double(tp.y);
+    }
+}
+
 // Transpose of the E update: lamH += sum of the hv/hu terms of the E edges that read each H.
 //   Ex reads Hz(p)-Hz(p-y), Hy(p)-Hy(p-z); Ey reads Hx(p)-Hx(p-z), Hz(p)-Hz(p-x);
 //   Ez reads Hy(p)-Hy(p-x), Hx(p)-Hx(p-y)
-__global__ __launch_bounds__(256) void k_adj_e(Geo g, const float *__restrict__ lamE, float *__restrict__ lamH,
-                                               const float *__restrict__ lp) {
+// so H at q gathers those terms at q and q+x, q+y, q+z. Then the step's gradient terms
+// lamE^{n+1} (E^n, curl^n) are accumulated on the design edges (lamE is read-only here).
+__global__ __launch_bounds__(256) void k_adj_e(Geo g, const float *__restrict__ la, const float *__restrict__ lamE,
+                                               float *__restrict__ lamH, const float *__restrict__ lp, GradArgs ga) {
     const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
     if (k >= g.n2 || j >= g.n1) return;
     const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
     float hv0, hu0, hv1, hu1, hv2, hu2, t, x0y, x0z, x1z, x1x, x2x, x2y;
-    e_terms<0>(g, lamE, lp, i, j, k, hv0, hu0);
-    e_terms<1>(g, lamE, lp, i, j, k, hv1, hu1);
-    e_terms<2>(g, lamE, lp, i, j, k, hv2, hu2);
-    e_terms<0>(g, lamE, lp, i, j + 1, k, x0y, t);  // hv of Ex at p+y
-    e_terms<0>(g, lamE, lp, i, j, k + 1, t, x0z);  // hu of Ex at p+z
-    e_terms<1>(g, lamE, lp, i, j, k + 1, x1z, t);  // hv of Ey at p+z
-    e_terms<1>(g, lamE, lp, i + 1, j, k, t, x1x);  // hu of Ey at p+x
-    e_terms<2>(g, lamE, lp, i + 1, j, k, x2x, t);  // hv of Ez at p+x
-    e_terms<2>(g, lamE, lp, i, j + 1, k, t, x2y);  // hu of Ez at p+y
+    e_terms<0>(g, la, lp, i, j, k, hv0, hu0);
+    e_terms<1>(g, la, lp, i, j, k, hv1, hu1);
+    e_terms<2>(g, la, lp, i, j, k, hv2, hu2);
+    e_terms<0>(g, la, lp, i, j + 1, k, x0y, t);  // hv of Ex at p+y
+    e_terms<0>(g, la, lp, i, j, k + 1, t, x0z);  // hu of Ex at p+z
+    e_terms<1>(g, la, lp, i, j, k + 1, x1z, t);  // hv of Ey at p+z
+    e_terms<1>(g, la, lp, i + 1, j, k, t, x1x);  // hu of Ey at p+x
+    e_terms<2>(g, la, lp, i + 1, j, k, x2x, t);  // hv of Ez at p+x
+    e_terms<2>(g, la, lp, i, j + 1, k, t, x2y);  // hu of Ez at p+y
     lamH[id] += (hv1 - x1z) + (hu2 - x2y);
     lamH[g.nn + id] += (hv2 - x2x) + (hu0 - x0z);
     lamH[2 * g.nn + id] += (hv0 - x0y) + (hu1 - x1x);
+    grad_terms(g, ga, lamE, i, j, k, id);
 }
 
 // psi adjoint of the node's own slots: psiBar_pre = be (psiBar + s) for an advanced slot, carried
@@ -138,38 +166,44 @@ __device__ __forceinline__ void psi_h_own(const Geo &g, const float *__restrict_
     }
 }
 
-// Transpose of the H update, and the rest of the E update's: lamE = ca lamE + the ev/eu terms of
-// the H components that read each E (Hx reads Ez(p+y)-Ez(p), Ey(p+z)-Ey(p); Hy reads
-// Ex(p+z)-Ex(p), Ez(p+x)-Ez(p); Hz reads Ey(p+x)-Ey(p), Ex(p+y)-Ex(p)); every psi adjoint
-// advances from lp into ln.
-__global__ __launch_bounds__(256) void k_adj_h(Geo g, float *__restrict__ lamE, const float *__restrict__ lamH,
-                                               const float *__restrict__ lp, float *__restrict__ ln) {
-    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
-    if (k >= g.n2 || j >= g.n1) return;
+// The observations whose dJ/ds enters lamE after this transpose (those of the step before).
+struct InjectArgs {
+    const uint64_t *obs;  // flat E index per observation
+    const double *ds;     // dJ/ds of the step before, NULL at the first step
+    uint32_t n_obs;
+};
+
+// Node (i,j,k)'s share of the H transpose, given the ev/eu terms at the node and at its -1
+// neighbours: lamE = ca lamE + gathered terms (+ dJ/ds where observed), la = cb lamE, and the
+// node's psi adjoints advanced into ln.
+__device__ __forceinline__ void adj_h_node(const Geo &g, float *__restrict__ lamE, float *__restrict__ la,
+                                           const float *__restrict__ lamH, const float *__restrict__ lp,
+                                           float *__restrict__ ln, const InjectArgs &in, uint32_t i, uint32_t j,
+                                           uint32_t k, const float t[6], const float y[6]) {
+    // t = {ev0, eu0, ev1, eu1, ev2, eu2} at p; y = {ev0(p-y), eu0(p-z), ev1(p-z), eu1(p-x), ev2(p-x), eu2(p-y)}
     const uint64_t nn = g.nn, id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
-    float ev0, eu0, ev1, eu1, ev2, eu2, t, y0y, y0z, y1z, y1x, y2x, y2y;
-    h_terms<0>(g, lamH, lp, i, j, k, ev0, eu0);
-    h_terms<1>(g, lamH, lp, i, j, k, ev1, eu1);
-    h_terms<2>(g, lamH, lp, i, j, k, ev2, eu2);
-    // -1 neighbours (an index of -1 wraps past the grid and reads as zero)
-    h_terms<0>(g, lamH, lp, i, j - 1, k, y0y, t);  // ev of Hx at p-y  (Hx: ev on Ez, u = y)
-    h_terms<0>(g, lamH, lp, i, j, k - 1, t, y0z);  // eu of Hx at p-z  (Hx: eu on Ey, v = z)
-    h_terms<1>(g, lamH, lp, i, j, k - 1, y1z, t);  // ev of Hy at p-z  (Hy: ev on Ex, u = z)
-    h_terms<1>(g, lamH, lp, i - 1, j, k, t, y1x);  // eu of Hy at p-x  (Hy: eu on Ez, v = x)
-    h_terms<2>(g, lamH, lp, i - 1, j, k, y2x, t);  // ev of Hz at p-x  (Hz: ev on Ey, u = x)
-    h_terms<2>(g, lamH, lp, i, j - 1, k, t, y2y);  // eu of Hz at p-y  (Hz: eu on Ex, v = y)
     float le[3] = {lamE[id], lamE[nn + id], lamE[2 * nn + id]};
-    float a[3];
+    float a[3], cb[3];
     bool on[3];
     for (int c = 0; c < 3; c++) {
         const float2 cf = coef_at(g, c * nn + id);
         on[c] = cf.y != 0.f;
+        cb[c] = cf.y;
         a[c] = cf.y * le[c];
         if (on[c]) le[c] *= cf.x;
     }
-    lamE[id] = le[0] + (y1z - ev1) + (y2y - eu2);
-    lamE[nn + id] = le[1] + (y2x - ev2) + (y0z - eu0);
-    lamE[2 * nn + id] = le[2] + (y0y - ev0) + (y1x - eu1);
+    le[0] += (y[2] - t[2]) + (y[5] - t[5]);  // Hy: ev on Ex (u = z); Hz: eu on Ex (v = y)
+    le[1] += (y[4] - t[4]) + (y[1] - t[1]);  // Hz: ev on Ey (u = x); Hx: eu on Ey (v = z)
+    le[2] += (y[0] - t[0]) + (y[3] - t[3]);  // Hx: ev on Ez (u = y); Hy: eu on Ez (v = x)
+    if (in.ds)
+        for (uint32_t q = 0; q < in.n_obs; q++) {
+            const uint64_t f = in.obs[q];
+            if (f - id < 3 * nn && (f - id) % nn == 0) le[(f - id) / nn] += float(in.ds[q]);
+        }
+    for (int c = 0; c < 3; c++) {
+        lamE[c * nn + id] = le[c];
+        la[c * nn + id] = cb[c] * le[c];
+    }
     if (g.pml) {
         psi_e_own<0>(g, a[0], on[0], lp, ln, i, j, k);
         psi_e_own<1>(g, a[1], on[1], lp, ln, i, j, k);
@@ -180,11 +214,38 @@ __global__ __launch_bounds__(256) void k_adj_h(Geo g, float *__restrict__ lamE, 
     }
 }
 
+// Transpose of the H update, and the rest of the E update's: lamE = ca lamE + the ev/eu terms of
+// the H components that read each E (Hx reads Ez(p+y)-Ez(p), Ey(p+z)-Ey(p); Hy reads
+// Ex(p+z)-Ex(p), Ez(p+x)-Ez(p); Hz reads Ey(p+x)-Ey(p), Ex(p+y)-Ex(p)), gathered at q and
+// q-x, q-y, q-z; every psi adjoint advances from lp into ln; then dJ/ds of the step before
+// enters lamE.
+__global__ __launch_bounds__(256) void k_adj_h(Geo g, float *__restrict__ lamE, float *__restrict__ la,
+                                               const float *__restrict__ lamH, const float *__restrict__ lp,
+                                               float *__restrict__ ln, InjectArgs in) {
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= g.n2 || j >= g.n1) return;
+    float t[6], y[6], u;
+    h_terms<0>(g, lamH, lp, i, j, k, t[0], t[1]);
+    h_terms<1>(g, lamH, lp, i, j, k, t[2], t[3]);
+    h_terms<2>(g, lamH, lp, i, j, k, t[4], t[5]);
+    h_terms<0>(g, lamH, lp, i, j - 1, k, y[0], u);  // ev of Hx at p-y
+    h_terms<0>(g, lamH, lp, i, j, k - 1, u, y[1]);  // eu of Hx at p-z
+    h_terms<1>(g, lamH, lp, i, j, k - 1, y[2], u);  // ev of Hy at p-z
+    h_terms<1>(g, lamH, lp, i - 1, j, k, u, y[3]);  // eu of Hy at p-x
+    h_terms<2>(g, lamH, lp, i - 1, j, k, y[4], u);  // ev of Hz at p-x
+    h_terms<2>(g, lamH, lp, i, j - 1, k, u, y[5]);  // eu of Hz at p-y
+    adj_h_node(g, lamE, la, lamH, lp, ln, in, i, j, k, t, y);
+}
+
 // dJ/ds_n enters lamE at the observed edges (s_n is E after step n)
-__global__ void k_adj_inject(float *__restrict__ lamE, const uint64_t *__restrict__ obs, uint32_t n_obs,
-                             const double *__restrict__ ds) {
-    const uint32_t q = threadIdx.x;
-    if (q < n_obs) lamE[obs[q]] += float(ds[q]);
+__global__ void k_adj_inject(Geo g, float *__restrict__ lamE, float *__restrict__ la,
+                             const uint64_t *__restrict__ obs, uint32_t n_obs, const double *__restrict__ ds) {
+    if (threadIdx.x) return;
+    for (uint32_t q = 0; q < n_obs; q++) {  // one thread: observations may share an edge
+        const uint64_t f = obs[q];
+        lamE[f] += float(ds[q]);
+        la[f] = coef_at(g, f).y * lamE[f];
+    }
 }
 
 // The forward step's E^n and curl^n = (t1 - t2) on the design edges (no CPML there), from the
@@ -208,17 +269,6 @@ __global__ __launch_bounds__(256) void k_adj_tape(Geo g, const uint64_t *__restr
         curl = (Hn[nn + id] - Hn[nn + id - sx]) * g.ax[0].inv_dd[i] - (Hn[id] - Hn[id - sy]) * g.ax[1].inv_dd[j];
     }
     tape[e] = make_float2(Eo[f], curl);
-}
-
-__global__ __launch_bounds__(256) void k_adj_grad(const uint64_t *__restrict__ edges, uint32_t ne,
-                                                  const float *__restrict__ lamE, const float2 *__restrict__ tape,
-                                                  double *__restrict__ gca, double *__restrict__ gcb) {
-    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= ne) return;
-    const double l = lamE[edges[e]];
-    const float2 t = tape[e];
-    gca[e] += l * double(t.x);
-    gcb[e] += l * double(t.y);
 }
 
 // ---- f64 forward (finite-difference checks) -------------------------------------------------
@@ -339,8 +389,9 @@ struct fg_adjoint {
     std::unique_ptr<fg::Session> s;
     std::vector<fg::DesignEdge> edges;
     bool built = false, have_forward = false;
-    fg::DevBuf<float> snaps, lamE, lamH, lamP[2];
+    fg::DevBuf<float> snaps, lamE, lamA, lamH, lamP[2];
     fg::DevBuf<uint64_t> d_edges, d_obs;
+    fg::DevBuf<int32_t> dmap;
     fg::DevBuf<float2> tape;
     fg::DevBuf<double> gca, gcb, dseries;
     uint64_t n_snaps = 0, snap_len = 0;
@@ -444,6 +495,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     const Geo g = geo(A);
     if (A.lamE.n != 3 * nn) {
         A.lamE.alloc(3 * nn);
+        A.lamA.alloc(3 * nn);
         A.lamH.alloc(3 * nn);
         A.lamP[0].alloc(std::max<uint64_t>(psi_n, 1));
         A.lamP[1].alloc(std::max<uint64_t>(psi_n, 1));
@@ -456,16 +508,36 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         A.tape.alloc(std::max<uint64_t>(A.every * ne, 1));
         A.gca.alloc(std::max<uint64_t>(ne, 1));
         A.gcb.alloc(std::max<uint64_t>(ne, 1));
+        // design edge per (component, node of the box's nodes lo..hi)
+        const fg_design &d = A.design;
+        uint64_t nb = 1;
+        for (int a = 0; a < 3; a++) nb *= d.hi[a] - d.lo[a] + 1;
+        std::vector<int32_t> map(3 * nb, -1);
+        for (uint64_t e = 0; e < ne; e++) {
+            const uint64_t c = f[e] / nn, id = f[e] % nn;
+            const uint64_t i = id / (uint64_t(s.n[1]) * s.n[2]), j = (id / s.n[2]) % s.n[1], k = id % s.n[2];
+            const uint64_t b = ((i - d.lo[0]) * (d.hi[1] - d.lo[1] + 1) + (j - d.lo[1])) * (d.hi[2] - d.lo[2] + 1) + (k - d.lo[2]);
+            map[c * nb + b] = int32_t(e);
+        }
+        A.dmap.upload(map.data(), map.size(), st);
     }
-    A.lamE.zero(st), A.lamH.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
+    A.lamE.zero(st), A.lamA.zero(st), A.lamH.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
     A.gca.zero(st), A.gcb.zero(st);
     A.dseries.upload(dseries, A.steps * n_obs, st);
     std::vector<hipEvent_t> ev(3 * A.n_snaps);
     for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
     const dim3 block(64, 4), grid((g.n2 + 63) / 64, (g.n1 + 3) / 4, g.n0);
     const uint32_t eb = uint32_t((ne + 255) / 256);
+    GradArgs ga{};
+    InjectArgs in{A.d_obs.p, nullptr, uint32_t(n_obs)};
+    for (int a = 0; a < 3; a++) ga.lo[a] = A.design.lo[a], ga.nb[a] = A.design.hi[a] - A.design.lo[a] + 1;
+    ga.map = A.dmap.p, ga.gca = A.gca.p, ga.gcb = A.gcb.p;
     int pin = 0;
     const auto t0 = std::chrono::steady_clock::now();
+    // dJ/ds of the last step; every earlier one enters at the end of the next step's transpose
+    if (n_obs && A.steps)
+        hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, st, g, A.lamE.p, A.lamA.p, A.d_obs.p, uint32_t(n_obs),
+                           A.dseries.p + (A.steps - 1) * n_obs);
     for (uint64_t k = A.n_snaps; k-- > 0;) {
         const uint64_t start = k * A.every, end = std::min(start + A.every, A.steps);
         FG_CHECK(hipEventRecord(ev[3 * k], st));
@@ -473,29 +545,29 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         if (k == 0) {
             s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
         } else {
-            const float *d = A.snaps.p + (k - 1) * A.snap_len;
-            FG_CHECK(hipMemcpyAsync(s.E[0].p, d, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
-            FG_CHECK(hipMemcpyAsync(s.H[0].p, d + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
-            if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, d + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
+            const float *src = A.snaps.p + (k - 1) * A.snap_len;
+            FG_CHECK(hipMemcpyAsync(s.E[0].p, src, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            FG_CHECK(hipMemcpyAsync(s.H[0].p, src + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+            if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, src + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
         }
         s.cur = 0;
         s.steps = start;
+        set_step_counter(s);
         for (uint64_t n = start; n < end; n++) {
             const int prev = s.cur;
-            enqueue_steps(s, 1);
-            hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
-                               s.H[s.cur].p, A.tape.p + (n - start) * ne);
+            launch_step(s, uint32_t(n - start), n);
+            if (ne)
+                hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
+                                   s.H[s.cur].p, A.tape.p + (n - start) * ne);
         }
+        s.steps = end;
         FG_CHECK(hipEventRecord(ev[3 * k + 1], st));
         for (uint64_t n = end; n-- > start;) {
-            if (n_obs)
-                hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64 * ((n_obs + 63) / 64)), 0, st, A.lamE.p, A.d_obs.p,
-                                   uint32_t(n_obs), A.dseries.p + n * n_obs);
-            if (ne)
-                hipLaunchKernelGGL(k_adj_grad, dim3(eb), dim3(256), 0, st, A.d_edges.p, uint32_t(ne), A.lamE.p,
-                                   A.tape.p + (n - start) * ne, A.gca.p, A.gcb.p);
-            hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamE.p, A.lamH.p, A.lamP[pin].p);
-            hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamH.p, A.lamP[pin].p, A.lamP[pin ^ 1].p);
+            ga.tape = A.tape.p + (n - start) * ne;
+            in.ds = n ? A.dseries.p + (n - 1) * n_obs : nullptr;
+            hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
+            hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamA.p, A.lamH.p, A.lamP[pin].p,
+                               A.lamP[pin ^ 1].p, in);
             pin ^= 1;
         }
         FG_CHECK(hipEventRecord(ev[3 * k + 2], st));
@@ -513,10 +585,10 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     // chain rule per design edge: ca = (1-x)/(1+x), cb = dt/(eps (1+x)), x = sig dt / (2 eps)
     // (sim.rs:431-441), eps = EPS0 (fixed + sum w eps_r)
-    std::vector<double> ga(ne), gb(ne);
+    std::vector<double> hca(ne), hcb(ne);
     if (ne) {
-        FG_CHECK(hipMemcpy(ga.data(), A.gca.p, ne * 8, hipMemcpyDeviceToHost));
-        FG_CHECK(hipMemcpy(gb.data(), A.gcb.p, ne * 8, hipMemcpyDeviceToHost));
+        FG_CHECK(hipMemcpy(hca.data(), A.gca.p, ne * 8, hipMemcpyDeviceToHost));
+        FG_CHECK(hipMemcpy(hcb.data(), A.gcb.p, ne * 8, hipMemcpyDeviceToHost));
     }
     std::fill(grad, grad + A.box_cells, 0.0);
     const double dt = fg_sim_dt(A.sim), eps0 = 8.8541878128e-12;
@@ -534,7 +606,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
             }
         const double eps = eps0 * er, x = de.sig * dt / (2.0 * eps);
         const double dca = 2.0 * x / (eps * (1.0 + x) * (1.0 + x)), dcb = -dt / (eps * eps * (1.0 + x) * (1.0 + x));
-        const double gedge = ga[e] * dca + gb[e] * dcb;
+        const double gedge = hca[e] * dca + hcb[e] * dcb;
         for (int q = 0; q < 4; q++)
             if (de.cell[q] != UINT32_MAX) grad[de.cell[q]] += gedge * eps0 * de.w[q];
     }

@@ -479,7 +479,7 @@ static void collect_timing(Session &s) {
 }
 
 // Enqueue one step at offset s_off from the device step counter.
-static void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
+void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     hipStream_t st = s.stream;
     const bool timed = s.timing;
     const size_t slot = 4 * size_t(s_off);
@@ -536,6 +536,10 @@ static void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 3), st));
     (void)global_step;
     FG_CHECK(hipGetLastError());
+}
+
+void set_step_counter(Session &s) {
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
 }
 
 void enqueue_steps(Session &s, uint64_t count) {

@@ -154,6 +154,10 @@ void xcd_order(std::vector<T> &v, size_t xcds, const T &pad) {
 
 // Enqueue `count` further steps on the session's stream (asynchronous).
 void enqueue_steps(Session &s, uint64_t count);
+// The pieces of enqueue_steps: load the device step counter with s.steps, then launch the
+// step at offset s_off from it (the caller advances s.steps afterwards).
+void set_step_counter(Session &s);
+void launch_step(Session &s, uint32_t s_off, uint64_t global_step);
 
 // Sum of E^2 + (376.73 H)^2 over the latest fields (cpu.rs:417-421); synchronizes the stream.
 double energy(Session &s);
