@@ -168,8 +168,8 @@ __device__ __forceinline__ void psi_h_own(const Geo &g, const float *__restrict_
 
 // The observations whose dJ/ds enters lamE after this transpose (those of the step before).
 struct InjectArgs {
-    const uint64_t *obs;  // flat E index per observation
-    const double *ds;     // dJ/ds of the step before, NULL at the first step
+    const uint64_t *__restrict__ obs;  // flat E index per observation
+    const double *__restrict__ ds;     // dJ/ds of the step before, NULL at the first step
     uint32_t n_obs;
 };
 
@@ -198,7 +198,8 @@ __device__ __forceinline__ void adj_h_node(const Geo &g, float *__restrict__ lam
     if (in.ds)
         for (uint32_t q = 0; q < in.n_obs; q++) {
             const uint64_t f = in.obs[q];
-            if (f - id < 3 * nn && (f - id) % nn == 0) le[(f - id) / nn] += float(in.ds[q]);
+            for (int c = 0; c < 3; c++)
+                if (f == c * nn + id) le[c] += float(in.ds[q]);
         }
     for (int c = 0; c < 3; c++) {
         lamE[c * nn + id] = le[c];
@@ -459,6 +460,7 @@ static void forward(fg_adjoint &A, float *series) {
     Session &s = *A.s;
     hipStream_t st = s.stream;
     reset(s);
+    s.observe = true;
     const uint64_t psi_n = s.psi[0].n, nn = s.nn;
     A.snap_len = 6 * nn + psi_n;
     A.n_snaps = (A.steps + A.every - 1) / A.every;  // segment k starts at k*every; k = 0 is all zeros
@@ -552,6 +554,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         }
         s.cur = 0;
         s.steps = start;
+        s.observe = false;  // the series of this segment is already known
         set_step_counter(s);
         for (uint64_t n = start; n < end; n++) {
             const int prev = s.cur;
@@ -561,6 +564,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
                                    s.H[s.cur].p, A.tape.p + (n - start) * ne);
         }
         s.steps = end;
+        s.observe = true;
         FG_CHECK(hipEventRecord(ev[3 * k + 1], st));
         for (uint64_t n = end; n-- > start;) {
             ga.tape = A.tape.p + (n - start) * ne;

@@ -531,7 +531,7 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
     if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, s.fix, Ecur, Eold, s_off);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 2), st));
-    if (s.obs_blocks)
+    if (s.obs_blocks && s.observe)
         hipLaunchKernelGGL(k_observe, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, Ecur, s.H[s.fused ? s.cur : 0].p, s_off);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 3), st));
     (void)global_step;

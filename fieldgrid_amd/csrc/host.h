@@ -85,6 +85,7 @@ struct Session {
     int cur = 0;  // which ping-pong copy holds the latest fields (fused)
     uint32_t fix_blocks = 0, obs_blocks = 0;
     bool timing = false;
+    bool observe = true;  // false: steps skip the observers (a recompute that already has them)
     double kernel_ms[3] = {0, 0, 0};  // step kernel(s), fix, observe
     uint64_t kernel_count = 0, pending_steps = 0;
     // buffers
