@@ -132,7 +132,7 @@ def sweep(args):
     seq = [F.run(gpu, r.sim, r.driven, r.pulse, r.extras) for r in runs]
     gpu.synchronize()
     seq_s = time.perf_counter() - t0
-    tune = F.Tune()
+    tune = F.Tune(chunk=args.graph_chunk)
     t0 = time.perf_counter()
     recs, spectra = F.run_batch(gpu, runs, tune=tune)
     gpu.synchronize()
@@ -240,6 +240,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="layered", choices=["layered", "boxes", "plain", "sweep", "adjoint"])
     ap.add_argument("--fd-cells", type=int, default=32, help="adjoint: design cells checked by central differences")
+    ap.add_argument("--graph-chunk", type=int, default=0,
+                    help="sweep: steps per captured hipGraph (0 = autotuned, 1 = direct launches)")
     ap.add_argument("--engine", default="fused", choices=["fused", "split"])
     ap.add_argument("--timed-steps", type=int, default=50, help="steps timed per kernel with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
