@@ -662,4 +662,363 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
     }
 }
 
+// ---- two steps per pass (temporal blocking) --------------------------------------------------
+//
+// march2 advances the fields by two steps in one pass over HBM: it reads step n-1's E, H, psi
+// from one ping-pong copy and writes step n+1's into the other; step n lives only in registers
+// and LDS. A block is TJ+4 rows x 64 lanes (j0-2..j0+TJ+1, k0-2..k0+61) and owns the inner
+// TJ x 60. The first sub-step (step n) runs on the whole tile, the second (step n+1) on the
+// tile less one ring, so the second sees exactly the step-n values the owners of its
+// neighbours computed (the same loads and the same arithmetic). Along i the block marches
+// with a two-plane lag: iteration t computes step n at plane t and step n+1 at plane t-1.
+// Step n's port sources are applied in the kernel, to every copy of the edge (owner or halo),
+// from a per-plane list; nothing else may sit between the two steps (no Debye media,
+// inductors, plane DFT or NTFF: those runs take one step per pass).
+constexpr int OWN_K2 = 60;
+
+// Step n's sources by plane: entries of plane i are ent[start[i] .. start[i+1]) as
+// (j, k, component, bits of src); the value subtracted is src * pulse(n).
+struct Src2 {
+    const uint32_t *start;
+    const uint4 *ent;
+};
+
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
+                                       float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
+                                       const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
+                                       const Src2 &sr, float pl) {
+    constexpr int ROWS = TJ + 4;
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    // row exchanges, double-buffered by plane parity; row w is stored at index w+1 so that
+    // rows -1 and ROWS read as zero pads. [0] E0 (Ex, Ez) for the row above's step-n H update,
+    // [1] step-n H (Hx, Hz) for the row below's step-n E update; [2], [3] the same for step n+1.
+    __shared__ float xr[4][2][ROWS + 2][2][64];
+    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
+    __shared__ float xa[SEG_MAX + 3][PXY ? 10 : 2];
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k = int(mb.bx) * OWN_K2 + lane - 2;
+    const int j = int(mb.by) * TJ + w - 2;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 2 && lane < 2 + OWN_K2 && w >= 2 && w < 2 + TJ;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    const uint32_t ob4 = off * 4u;
+    const uint32_t ldA = act ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
+    if constexpr (PML) {
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
+    }
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    // step n runs on planes tstart..i1, step n+1 on planes max(i0,1)-1 .. i1-1 (the plane
+    // before i0 feeds the -x neighbour of i0's E update)
+    const uint32_t tstart = i0 >= 2 ? i0 - 2 : 0;
+    for (uint32_t t = w * 64 + lane; t + tstart <= i1; t += 64 * ROWS) {
+        const uint32_t q = min(tstart + t, g.n0 - 1);
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        if constexpr (PXY) {
+            xa[t][2] = g.ax[0].bh[q], xa[t][3] = g.ax[0].ch[q], xa[t][4] = g.ax[0].raw_d[q];
+            xa[t][5] = g.ax[0].be[q], xa[t][6] = g.ax[0].ce[q], xa[t][7] = g.ax[0].raw_dd[q];
+            xa[t][8] = __int_as_float(g.ax[0].slot_h[q]), xa[t][9] = __int_as_float(g.ax[0].slot_e[q]);
+        }
+    }
+    if (w < 2)
+        for (int x = 0; x < 4; x++)
+            for (int q = 0; q < 2; q++) xr[x][q][w * (ROWS + 1)][0][lane] = 0.f, xr[x][q][w * (ROWS + 1)][1][lane] = 0.f;
+    // plane p of a field array through a resource that is empty when p is outside the grid
+    auto pbase = [&](const float *a, int64_t p) { return a + (p >= 0 && p < int64_t(g.n0) ? uint64_t(p) * plane : 0); };
+    auto pbytes = [&](int64_t p, uint32_t bytes) { return p >= 0 && p < int64_t(g.n0) ? bytes : 0u; };
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    // carried state: E0 of the current plane, step-n values of the previous plane, step-n+1 H
+    // of the plane before, coefficient pairs and step-n psi of the previous plane
+    float ex0, ey0, ez0;
+    {
+        const float *p = pbase(Eo, tstart);
+        const uint32_t b = pbytes(tstart, pl4);
+        ex0 = ld(p, ldA, b), ey0 = ld(p + nn, ldA, b), ez0 = ld(p + 2 * nn, ldA, b);
+    }
+    {
+        const int pb = (tstart + 1) & 1;  // the buffer the first iteration reads
+        xr[0][pb][w + 1][0][lane] = ex0, xr[0][pb][w + 1][1][lane] = ez0;
+    }
+    __syncthreads();
+    float h1x = 0, h1y = 0, h1z = 0;           // step-n H at the previous plane
+    float e1x = 0, e1y = 0, e1z = 0;           // step-n E at the previous plane
+    float h2y = 0, h2z = 0;                    // step-n+1 Hy, Hz at the plane before that
+    float2 cf0 = {0, 0}, cf1 = {0, 0}, cf2 = {0, 0};  // (ca, cb) of the previous plane
+    // step-n psi of the previous plane (component, axis)
+    float q_hxy = 0, q_hxz = 0, q_hyz = 0, q_hyx = 0, q_hzx = 0, q_hzy = 0;
+    float q_exy = 0, q_exz = 0, q_eyz = 0, q_eyx = 0, q_ezx = 0, q_ezy = 0;
+    for (uint32_t t = tstart; t <= i1; t++) {
+        const int b = t & 1;
+        const uint64_t pb = uint64_t(t) * plane;
+        const bool il = t + 1 < g.n0;
+        // loads of plane t: E0(t+1), H0(t), coefficient indices, step n-1 psi
+        const float *pe = pbase(Eo, int64_t(t) + 1);
+        const uint32_t eb = pbytes(int64_t(t) + 1, pl4);
+        const float exn = ld(pe, ldA, eb), eyn = ld(pe + nn, ldA, eb), ezn = ld(pe + 2 * nn, ldA, eb);
+        const float *ph = pbase(Ho, t);
+        const uint32_t hb = pbytes(t, pl4);
+        const float hx0 = ld(ph, ldA, hb), hy0 = ld(ph + nn, ldA, hb), hz0 = ld(ph + 2 * nn, ldA, hb);
+        const uint32_t cbytes = pbytes(t, plc);
+        const uint64_t cb_ = t < g.n0 ? pb : 0;
+        const uint32_t c0 = cidx(cb_, cbytes), c1 = cidx(nn + cb_, cbytes), c2 = cidx(2 * nn + cb_, cbytes);
+        const float *xv = xa[t - tstart];
+        const float idx_ = xv[0], iddx = xv[1];
+        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
+        float hxy = 0, hxz = 0, hyz = 0, hyx = 0, hzx = 0, hzy = 0;
+        float exy = 0, exz = 0, eyz = 0, eyx = 0, ezx = 0, ezy = 0;
+        uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
+        uint32_t oxh = OOB, oxe = OOB;
+        const uint32_t zb = t < g.n0 ? zbytes : 0u, yb = t < g.n0 ? ybytes : 0u, xb = t < g.n0 ? pl4 : 0u;
+        if constexpr (PZ) {
+            zrow = t < g.n0 ? uint64_t(t) * g.n1 * p2 : 0;
+            hxz = ld(psio + PSI(1, 0, 2) + zrow, ozh, zb), hyz = ld(psio + PSI(1, 1, 2) + zrow, ozh, zb);
+            exz = ld(psio + PSI(0, 0, 2) + zrow, oze, zb), eyz = ld(psio + PSI(0, 1, 2) + zrow, oze, zb);
+        }
+        if constexpr (PXY) {
+            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
+            bhx = xv[2], chx = xv[3], rdx = xv[4];
+            bex = xv[5], cex = xv[6], rddx = xv[7];
+            yrow = t < g.n0 ? uint64_t(t) * p2 * g.n2 : 0;
+            xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
+            oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
+            hxy = ld(psio + PSI(1, 0, 1) + yrow, oyh, yb);
+            hyx = ld(psio + PSI(1, 1, 0) + xhb, oxh, xb);
+            hzx = ld(psio + PSI(1, 2, 0) + xhb, oxh, xb), hzy = ld(psio + PSI(1, 2, 1) + yrow, oyh, yb);
+            exy = ld(psio + PSI(0, 0, 1) + yrow, oye, yb);
+            eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, xb);
+            ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, xb), ezy = ld(psio + PSI(0, 2, 1) + yrow, oye, yb);
+        }
+        // ---- step n, H at plane t (the +1 neighbours of E0: j+1 from LDS, k+1 from the next lane)
+        const float ex_j = xr[0][b ^ 1][w + 2][0][lane], ez_j = xr[0][b ^ 1][w + 2][1][lane];
+        const float ex_k = __shfl_down(ex0, 1, 64), ey_k = __shfl_down(ey0, 1, 64);
+        float nhx, nhy, nhz;
+        {
+            const float ev = ez_j - ez0, eu = ey_k - ey0;
+            float t1 = ev * idy, t2 = eu * idz;
+            if constexpr (PXY) hxy = bhy * hxy + chy * ev * rdy, t1 += hxy;
+            if constexpr (PZ) hxz = bhz * hxz + chz * eu * rdz, t2 += hxz;
+            nhx = jl && kl ? hx0 - g.k_mu * (t1 - t2) : hx0;
+        }
+        {
+            const float ev = ex_k - ex0, eu = ezn - ez0;
+            float t1 = ev * idz, t2 = eu * idx_;
+            if constexpr (PZ) hyz = bhz * hyz + chz * ev * rdz, t1 += hyz;
+            if constexpr (PXY) hyx = bhx * hyx + chx * eu * rdx, t2 += hyx;
+            nhy = kl && il ? hy0 - g.k_mu * (t1 - t2) : hy0;
+        }
+        {
+            const float ev = eyn - ey0, eu = ex_j - ex0;
+            float t1 = ev * idx_, t2 = eu * idy;
+            if constexpr (PXY) hzx = bhx * hzx + chx * ev * rdx, t1 += hzx;
+            if constexpr (PXY) hzy = bhy * hzy + chy * eu * rdy, t2 += hzy;
+            nhz = il && jl ? hz0 - g.k_mu * (t1 - t2) : hz0;
+        }
+        xr[1][b][w + 1][0][lane] = nhx, xr[1][b][w + 1][1][lane] = nhz;
+        xr[0][b][w + 1][0][lane] = exn, xr[0][b][w + 1][1][lane] = ezn;
+        __syncthreads();
+        // ---- step n, E at plane t (the -1 neighbours of step-n H: j-1 from LDS, k-1 from the
+        // previous lane, i-1 from the previous iteration)
+        const float hx_j = xr[1][b][w][0][lane], hz_j = xr[1][b][w][1][lane];
+        const float hx_k = __shfl_up(nhx, 1, 64), hy_k = __shfl_up(nhy, 1, 64);
+        const float2 f0 = coef(c0), f1 = coef(c1), f2 = coef(c2);
+        float nex, ney, nez;
+        {
+            const float hv = nhz - hz_j, hu = nhy - hy_k;
+            float t1 = hv * iddy, t2 = hu * iddz;
+            if constexpr (PXY) exy = bey * exy + cey * hv * rddy, t1 += exy;
+            if constexpr (PZ) exz = bez * exz + cez * hu * rddz, t2 += exz;
+            nex = f0.y != 0.f ? f0.x * ex0 + f0.y * (t1 - t2) : ex0;
+        }
+        {
+            const float hv = nhx - hx_k, hu = nhz - h1z;
+            float t1 = hv * iddz, t2 = hu * iddx;
+            if constexpr (PZ) eyz = bez * eyz + cez * hv * rddz, t1 += eyz;
+            if constexpr (PXY) eyx = bex * eyx + cex * hu * rddx, t2 += eyx;
+            ney = f1.y != 0.f ? f1.x * ey0 + f1.y * (t1 - t2) : ey0;
+        }
+        {
+            const float hv = nhy - h1y, hu = nhx - hx_j;
+            float t1 = hv * iddx, t2 = hu * iddy;
+            if constexpr (PXY) ezx = bex * ezx + cex * hv * rddx, t1 += ezx;
+            if constexpr (PXY) ezy = bey * ezy + cey * hu * rddy, t2 += ezy;
+            nez = f2.y != 0.f ? f2.x * ez0 + f2.y * (t1 - t2) : ez0;
+        }
+        // step n's port sources on this plane (every copy of the edge, owner or halo)
+        if (t < g.n0) {
+            const uint32_t s0 = sr.start[t], s1 = sr.start[t + 1];
+            for (uint32_t q = s0; q < s1; q++) {
+                const uint4 e = sr.ent[q];
+                if (int(e.x) == j && int(e.y) == k) {
+                    const float d = __uint_as_float(e.w) * pl;
+                    if (e.z == 0) nex -= d;
+                    else if (e.z == 1) ney -= d;
+                    else nez -= d;
+                }
+            }
+        }
+        // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)
+        const bool two = t >= 1 && t - 1 + 1 >= (i0 > 0 ? i0 : 1);
+        const uint32_t s = t - 1;  // plane of step n+1
+        const bool sl = t < g.n0;  // s+1 inside the grid (the i+1 neighbour exists)
+        float mhy = 0.f, mhz = 0.f;
+        if (two) {
+            const float *xs = xa[s - tstart];
+            const float sidx = xs[0], siddx = xs[1];
+            float sbhx = 0, schx = 0, srdx = 0, sbex = 0, scex = 0, srddx = 0;
+            uint32_t soxh = OOB, soxe = OOB;
+            uint64_t szrow = 0, syrow = 0, sxhb = 0, sxeb = 0;
+            if constexpr (PZ) szrow = uint64_t(s) * g.n1 * p2;
+            if constexpr (PXY) {
+                const int shx = __float_as_int(xs[8]), sex = __float_as_int(xs[9]);
+                sbhx = xs[2], schx = xs[3], srdx = xs[4];
+                sbex = xs[5], scex = xs[6], srddx = xs[7];
+                syrow = uint64_t(s) * p2 * g.n2;
+                sxhb = uint64_t(max(shx, 0)) * plane, sxeb = uint64_t(max(sex, 0)) * plane;
+                soxh = shx >= 0 ? ob4 : OOB, soxe = sex >= 0 ? ob4 : OOB;
+            }
+            const bool wr = own && s >= i0;  // this thread stores plane s
+            const uint32_t sto = wr ? stO : OOB;
+            const float ex_j2 = xr[2][b ^ 1][w + 2][0][lane], ez_j2 = xr[2][b ^ 1][w + 2][1][lane];
+            const float ex_k2 = __shfl_down(e1x, 1, 64), ey_k2 = __shfl_down(e1y, 1, 64);
+            float ohx, ohy, ohz;
+            {
+                const float ev = ez_j2 - e1z, eu = ey_k2 - e1y;
+                float t1 = ev * idy, t2 = eu * idz;
+                const bool v = jl && kl;
+                if constexpr (PXY) {
+                    q_hxy = bhy * q_hxy + chy * ev * rdy, t1 += q_hxy;
+                    st(psin + PSI(1, 0, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hxy);
+                }
+                if constexpr (PZ) {
+                    q_hxz = bhz * q_hxz + chz * eu * rdz, t2 += q_hxz;
+                    st(psin + PSI(1, 0, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hxz);
+                }
+                ohx = v ? h1x - g.k_mu * (t1 - t2) : h1x;
+            }
+            {
+                const float ev = ex_k2 - e1x, eu = nez - e1z;
+                float t1 = ev * idz, t2 = eu * sidx;
+                const bool v = kl && sl;
+                if constexpr (PZ) {
+                    q_hyz = bhz * q_hyz + chz * ev * rdz, t1 += q_hyz;
+                    st(psin + PSI(1, 1, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hyz);
+                }
+                if constexpr (PXY) {
+                    q_hyx = sbhx * q_hyx + schx * eu * srdx, t2 += q_hyx;
+                    st(psin + PSI(1, 1, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hyx);
+                }
+                ohy = v ? h1y - g.k_mu * (t1 - t2) : h1y;
+            }
+            {
+                const float ev = ney - e1y, eu = ex_j2 - e1x;
+                float t1 = ev * sidx, t2 = eu * idy;
+                const bool v = sl && jl;
+                if constexpr (PXY) {
+                    q_hzx = sbhx * q_hzx + schx * ev * srdx, t1 += q_hzx;
+                    st(psin + PSI(1, 2, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hzx);
+                    q_hzy = bhy * q_hzy + chy * eu * rdy, t2 += q_hzy;
+                    st(psin + PSI(1, 2, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hzy);
+                }
+                ohz = v ? h1z - g.k_mu * (t1 - t2) : h1z;
+            }
+            {
+                float *q = Hn + uint64_t(s) * plane;
+                st(q, sto, pl4, ohx), st(q + nn, sto, pl4, ohy), st(q + 2 * nn, sto, pl4, ohz);
+            }
+            xr[3][b][w + 1][0][lane] = ohx, xr[3][b][w + 1][1][lane] = ohz;
+            xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
+            __syncthreads();
+            const float hx_j2 = xr[3][b][w][0][lane], hz_j2 = xr[3][b][w][1][lane];
+            const float hx_k2 = __shfl_up(ohx, 1, 64), hy_k2 = __shfl_up(ohy, 1, 64);
+            float *q = En + uint64_t(s) * plane;
+            {
+                const float hv = ohz - hz_j2, hu = ohy - hy_k2;
+                float t1 = hv * iddy, t2 = hu * iddz;
+                const bool on = cf0.y != 0.f;
+                if constexpr (PXY) {
+                    q_exy = bey * q_exy + cey * hv * rddy, t1 += q_exy;
+                    st(psin + PSI(0, 0, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_exy);
+                }
+                if constexpr (PZ) {
+                    q_exz = bez * q_exz + cez * hu * rddz, t2 += q_exz;
+                    st(psin + PSI(0, 0, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_exz);
+                }
+                st(q, sto, pl4, on ? cf0.x * e1x + cf0.y * (t1 - t2) : e1x);
+            }
+            {
+                const float hv = ohx - hx_k2, hu = ohz - h2z;
+                float t1 = hv * iddz, t2 = hu * siddx;
+                const bool on = cf1.y != 0.f;
+                if constexpr (PZ) {
+                    q_eyz = bez * q_eyz + cez * hv * rddz, t1 += q_eyz;
+                    st(psin + PSI(0, 1, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_eyz);
+                }
+                if constexpr (PXY) {
+                    q_eyx = sbex * q_eyx + scex * hu * srddx, t2 += q_eyx;
+                    st(psin + PSI(0, 1, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_eyx);
+                }
+                st(q + nn, sto, pl4, on ? cf1.x * e1y + cf1.y * (t1 - t2) : e1y);
+            }
+            {
+                const float hv = ohy - h2y, hu = ohx - hx_j2;
+                float t1 = hv * siddx, t2 = hu * iddy;
+                const bool on = cf2.y != 0.f;
+                if constexpr (PXY) {
+                    q_ezx = sbex * q_ezx + scex * hv * srddx, t1 += q_ezx;
+                    st(psin + PSI(0, 2, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_ezx);
+                    q_ezy = bey * q_ezy + cey * hu * rddy, t2 += q_ezy;
+                    st(psin + PSI(0, 2, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_ezy);
+                }
+                st(q + 2 * nn, sto, pl4, on ? cf2.x * e1z + cf2.y * (t1 - t2) : e1z);
+            }
+            mhy = ohy, mhz = ohz;
+        } else {
+            xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
+            __syncthreads();
+        }
+        // rotate: plane t becomes the previous plane
+        h2y = mhy, h2z = mhz;
+        h1x = nhx, h1y = nhy, h1z = nhz;
+        e1x = nex, e1y = ney, e1z = nez;
+        cf0 = f0, cf1 = f1, cf2 = f2;
+        q_hxy = hxy, q_hxz = hxz, q_hyz = hyz, q_hyx = hyx, q_hzx = hzx, q_hzy = hzy;
+        q_exy = exy, q_exz = exz, q_eyz = eyz, q_eyx = eyx, q_ezx = ezx, q_ezy = ezy;
+        ex0 = exn, ey0 = eyn, ez0 = ezn;
+    }
+#undef PSI
+}
+
 }  // namespace fg

@@ -95,6 +95,110 @@ __global__ __launch_bounds__(256) void k_observe(Geo g, ObsArgs a, const float *
     observe(g, a, E, H, s_off, blockIdx.x);
 }
 
+// Two steps of the march per launch (see march2): the first step's pulse value comes from the
+// device step counter and the host table, as in k_fix.
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBlock *__restrict__ blocks,
+                                                          const float *__restrict__ Eo, float *__restrict__ En,
+                                                          const float *__restrict__ Ho, float *__restrict__ Hn,
+                                                          const float *__restrict__ psio, float *__restrict__ psin,
+                                                          uint32_t seg, Src2 sr, FixArgs f, uint32_t s_off) {
+    const MarchBlock mb = blocks[blockIdx.x];
+    if (mb.bx == ~0u) return;
+    const uint32_t n = f.step[0] + s_off;
+    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
+    march2<TJ, PML, WIDE, LDS_SETS>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl);
+}
+
+// The first step of a two-step launch exists only inside it. Its observers (port V/I, point
+// probes) recompute the few values they read from the untouched previous copy, with the
+// march's arithmetic: H by h_next, E by the same update, then the driven port's sources.
+template <int C>
+__device__ float h_mid(const Geo &g, const float *__restrict__ E, const float *__restrict__ H,
+                       const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
+    return h_next<C>(g, E, H, psi, nullptr, i, j, k);
+}
+
+template <int C>
+__device__ float e_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
+                       const float *__restrict__ H, const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const float eold = E[C * g.nn + id];
+    const float2 cf = coef_at(g, C * g.nn + id);
+    float e = eold;
+    if (cf.y != 0.f) {
+        uint32_t um[3] = {i, j, k}, vm[3] = {i, j, k};
+        um[U] -= 1;
+        vm[V] -= 1;
+        const float hv = h_mid<V>(g, E, H, psi, i, j, k) - h_mid<V>(g, E, H, psi, um[0], um[1], um[2]);
+        const float hu = h_mid<U>(g, E, H, psi, i, j, k) - h_mid<U>(g, E, H, psi, vm[0], vm[1], vm[2]);
+        const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+        float t1 = hv * g.ax[U].inv_dd[pu], t2 = hu * g.ax[V].inv_dd[pv];
+        if (g.pml) {
+            const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
+            if (su >= 0)
+                t1 += g.ax[U].be[pu] * psi[psi_at<U>(g, 0, C, i, j, k, uint32_t(su))] + g.ax[U].ce[pu] * hv * g.ax[U].raw_dd[pu];
+            if (sv >= 0)
+                t2 += g.ax[V].be[pv] * psi[psi_at<V>(g, 0, C, i, j, k, uint32_t(sv))] + g.ax[V].ce[pv] * hu * g.ax[V].raw_dd[pv];
+        }
+        e = cf.x * eold + cf.y * (t1 - t2);
+    }
+    for (uint32_t q = 0; q < f.n_src; q++)
+        if (f.src[q].comp == C && f.src[q].id == id) e -= f.src[q].src * pl;
+    return e;
+}
+
+__device__ float field_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
+                           const float *__restrict__ H, const float *__restrict__ psi, uint64_t flat) {
+    const uint32_t c = uint32_t(flat / g.nn);
+    const uint64_t id = flat - c * g.nn;
+    const uint32_t k = uint32_t(id % g.n2), j = uint32_t((id / g.n2) % g.n1), i = uint32_t(id / (uint64_t(g.n1) * g.n2));
+    switch (c) {
+        case 0: return e_mid<0>(g, f, pl, E, H, psi, i, j, k);
+        case 1: return e_mid<1>(g, f, pl, E, H, psi, i, j, k);
+        case 2: return e_mid<2>(g, f, pl, E, H, psi, i, j, k);
+        case 3: return h_mid<0>(g, E, H, psi, i, j, k);
+        case 4: return h_mid<1>(g, E, H, psi, i, j, k);
+        default: return h_mid<2>(g, E, H, psi, i, j, k);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_observe_mid(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ E,
+                                                     const float *__restrict__ H, const float *__restrict__ psi,
+                                                     uint32_t s_off) {
+    const uint32_t n = a.step[0] + s_off;
+    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
+    const uint64_t nn = g.nn;
+    uint32_t blk = blockIdx.x;
+    if (blk < a.blk_ports) {  // as observe(): one wave per port, f32 products summed in f64
+        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+        if (p >= a.n_ports) return;
+        const fg_port d = a.ports[p];
+        double v = 0.0, cur = 0.0;
+        for (uint32_t q = lane; q < d.n_edges; q += 64) {
+            const fg_port_edge pe = a.edges[d.first_edge + q];
+            v -= double(field_mid(g, f, pl, E, H, psi, d.comp * nn + pe.id) * pe.w);
+        }
+        for (uint32_t q = lane; q < d.n_loops; q += 64) {
+            const fg_loop_term lt = a.loops[d.first_loop + q];
+            cur += double(field_mid(g, f, pl, E, H, psi, (3 + lt.comp) * nn + lt.id) * lt.w);
+        }
+        v = wave_sum(v);
+        cur = wave_sum(cur);
+        if (lane == 0 && n < a.cap) {
+            a.series[(uint64_t(n) * a.n_ports + p) * 2] = float(v);
+            a.series[(uint64_t(n) * a.n_ports + p) * 2 + 1] = float(cur);
+        }
+        return;
+    }
+    blk -= a.blk_ports;
+    if (blk < a.blk_probes) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = field_mid(g, f, pl, E, H, psi, a.probe_f[q]);
+    }
+}
+
 // ---- energy: sum E^2 + (376.73 H)^2 (cpu.rs:417-421, fdtd.wgsl:547-568) -----------------
 
 __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ E, const float *__restrict__ H, uint64_t total,
@@ -194,6 +298,44 @@ void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<Marc
                 const bool xy = touches(1, j0 - 1, j0 + tj) || touches(0, i0 - 1, i0 + seg);
                 out[xy ? 2 : z ? 1 : 0].push_back(MarchBlock{bx, by, bz});
             }
+}
+
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]) {
+    const uint32_t nmax = s.nmax;
+    std::vector<uint32_t> pre[3];
+    for (int a = 0; a < 3; a++) {
+        pre[a].assign(s.n[a] + 1, 0);
+        for (uint32_t q = 0; q < s.n[a]; q++) {
+            const bool slab = s.pml && (s.slots_host[(size_t(a) * 2) * nmax + q] >= 0 ||
+                                        s.slots_host[(size_t(a) * 2 + 1) * nmax + q] >= 0);
+            pre[a][q + 1] = pre[a][q] + (slab ? 1 : 0);
+        }
+    }
+    auto touches = [&](int a, int64_t lo, int64_t hi) {
+        lo = std::max<int64_t>(lo, 0);
+        hi = std::min<int64_t>(hi, int64_t(s.n[a]) - 1);
+        return hi >= lo && pre[a][size_t(hi) + 1] > pre[a][size_t(lo)];
+    };
+    const uint32_t gx = (s.n[2] + OWN_K2 - 1) / OWN_K2, gy = (s.n[1] + tj - 1) / tj, gz = (s.n[0] + seg - 1) / seg;
+    for (int c = 0; c < 3; c++) out[c].clear();
+    for (uint32_t bz = 0; bz < gz; bz++)
+        for (uint32_t by = 0; by < gy; by++)
+            for (uint32_t bx = 0; bx < gx; bx++) {
+                const int64_t k0 = int64_t(bx) * OWN_K2, j0 = int64_t(by) * tj, i0 = int64_t(bz) * seg;
+                const int64_t i1 = std::min<int64_t>(i0 + seg, s.n[0]);
+                const bool z = touches(2, k0 - 2, k0 + OWN_K2 + 1);
+                const bool xy = touches(1, j0 - 2, j0 + tj + 1) || touches(0, i0 - 2, i1);
+                out[xy ? 2 : z ? 1 : 0].push_back(MarchBlock{bx, by, bz});
+            }
+}
+
+static void plan_march2(Session &s) {
+    std::vector<MarchBlock> cls[3];
+    march_classes2(s, s.tj2, s.seg2, cls);
+    for (int c = 0; c < 3; c++) xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
+    s.blk2_int.upload(cls[0].data(), cls[0].size(), s.stream);
+    s.blk2_z.upload(cls[1].data(), cls[1].size(), s.stream);
+    s.blk2_pml.upload(cls[2].data(), cls[2].size(), s.stream);
 }
 
 static void plan_march(Session &s) {
@@ -431,6 +573,42 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     s.nmax = nmax;
     s.slots_host = axs;
     if (s.fused) plan_march(s);
+    // two steps per launch when nothing but port sources acts between steps
+    const char *f2 = getenv("FIELDGRID_FUSE2");
+    s.fuse2 = s.fused && !(f2 && f2[0] == '0') && f.n_debye == 0 && f.n_ind == 0 && !f.serial && o.plane_n == 0 &&
+              o.n_patches == 0;
+    if (s.fuse2) {
+        s.seg2 = 32;
+        if (const char *v = getenv("FIELDGRID_SEG2")) s.seg2 = std::max(1u, uint32_t(atoi(v)));
+        s.seg2 = std::min<uint32_t>(s.seg2, SEG_MAX);
+        if (!getenv("FIELDGRID_SEG2"))
+            while (s.seg2 > 4 && uint64_t((P->n[2] + OWN_K2 - 1) / OWN_K2) * ((P->n[1] + s.tj2 - 1) / s.tj2) *
+                                     ((P->n[0] + s.seg2 - 1) / s.seg2) < 3ull * uint64_t(std::max(dev->cus, 1)))
+                s.seg2 /= 2;
+        // the driven port's sources by plane (i), for the first step inside the launch
+        std::vector<std::vector<uint4>> by_plane(P->n[0]);
+        if (P->n_ports)
+            for (uint32_t q = 0; q < f.n_src; q++) {
+                const fg_source &e = P->sources[P->ports[driven].first_source + q];
+                const uint64_t id = e.id;
+                const uint32_t i = uint32_t(id / (uint64_t(P->n[1]) * P->n[2])), j = uint32_t((id / P->n[2]) % P->n[1]),
+                               k = uint32_t(id % P->n[2]);
+                uint32_t bits;
+                std::memcpy(&bits, &e.src, 4);
+                by_plane[i].push_back(make_uint4(j, k, e.comp, bits));
+            }
+        std::vector<uint32_t> start(P->n[0] + 1, 0);
+        std::vector<uint4> ent;
+        for (uint32_t i = 0; i < P->n[0]; i++) {
+            start[i] = uint32_t(ent.size());
+            ent.insert(ent.end(), by_plane[i].begin(), by_plane[i].end());
+        }
+        start[P->n[0]] = uint32_t(ent.size());
+        s.src2_start.upload(start.data(), start.size(), st);
+        if (ent.empty()) s.src2_ent.alloc(1);
+        else s.src2_ent.upload(ent.data(), ent.size(), st);
+        plan_march2(s);
+    }
     FG_CHECK(hipStreamSynchronize(st));
     return S.release();
 }
@@ -466,16 +644,17 @@ static hipEvent_t timing_event(Session &s, size_t slot) {
 
 // Fold the recorded event pairs of the last enqueue into the per-kernel-group totals.
 static void collect_timing(Session &s) {
-    if (!s.timing || s.pending_steps == 0) return;
+    if (!s.timing || s.units.empty()) return;
     FG_CHECK(hipStreamSynchronize(s.stream));
-    for (uint64_t q = 0; q < s.pending_steps; q++)
+    for (const auto &u : s.units) {
         for (int g = 0; g < 3; g++) {
             float ms = 0.f;
-            FG_CHECK(hipEventElapsedTime(&ms, s.events[4 * q + g], s.events[4 * q + g + 1]));
+            FG_CHECK(hipEventElapsedTime(&ms, s.events[u.first + g], s.events[u.first + g + 1]));
             s.kernel_ms[g] += ms;
         }
-    s.kernel_count += s.pending_steps;
-    s.pending_steps = 0;
+        s.kernel_count += u.second;
+    }
+    s.units.clear();
 }
 
 // Enqueue one step at offset s_off from the device step counter.
@@ -484,6 +663,7 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     const bool timed = s.timing;
     const size_t slot = 4 * size_t(s_off);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot), st));
+    if (timed) s.units.push_back({slot, 1u});
     const Geo &g = s.geo;
     const int a = s.cur, b = 1 - s.cur;
     float *Ecur;
@@ -538,6 +718,59 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     FG_CHECK(hipGetLastError());
 }
 
+// Enqueue two steps (s_off and s_off + 1) with one launch per block class of march2: the first
+// step's observers recompute from the copy the launch reads, the second's run as usual after
+// its sparse fixes.
+static void launch_pair(Session &s, uint32_t s_off) {
+    hipStream_t st = s.stream;
+    const bool timed = s.timing;
+    const size_t slot = 4 * size_t(s_off);
+    if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot), st));
+    if (timed) s.units.push_back({slot, 2u});
+    const Geo &g = s.geo;
+    const int a = s.cur, b = 1 - s.cur;
+    const dim3 block(64, s.tj2 + 4);
+    const float *Eo = s.E[a].p, *Ho = s.H[a].p, *Po = s.psi[a].p;
+    float *En = s.E[b].p, *Hn = s.H[b].p, *Pn = s.psi[b].p;
+    const uint32_t ni = uint32_t(s.blk2_int.n), nz = uint32_t(s.blk2_z.n), np_ = uint32_t(s.blk2_pml.n);
+    const MarchBlock *bi = s.blk2_int.p, *bz = s.blk2_z.p, *bp = s.blk2_pml.p;
+    const Src2 sr{s.src2_start.p, s.src2_ent.p};
+#define FG_MARCH2X(TJ, W, L)                                                                                         \
+    do {                                                                                                             \
+        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
+        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
+        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
+    } while (0)
+    {
+        const bool lds = g.n_sets <= uint32_t(SETS_LDS);
+        if (g.wide) {
+            if (lds) FG_MARCH2X(12, true, true); else FG_MARCH2X(12, true, false);
+        } else {
+            if (lds) FG_MARCH2X(12, false, true); else FG_MARCH2X(12, false, false);
+        }
+    }
+#undef FG_MARCH2X
+    s.cur = b;
+    if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
+    if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, s.fix, En, Eo, s_off + 1);
+    if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 2), st));
+    if (s.obs_blocks && s.observe) {
+        hipLaunchKernelGGL(k_observe_mid, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, s_off);
+        hipLaunchKernelGGL(k_observe, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, En, Hn, s_off + 1);
+    }
+    if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 3), st));
+    FG_CHECK(hipGetLastError());
+}
+
+// Steps [s.steps, s.steps + count) from offset 0 of the device counter: pairs through march2
+// where it is on, single steps otherwise (and for an odd last one).
+static void launch_steps(Session &s, uint64_t count) {
+    uint64_t q = 0;
+    if (s.fuse2)
+        for (; q + 2 <= count; q += 2) launch_pair(s, uint32_t(q));
+    for (; q < count; q++) launch_step(s, uint32_t(q), s.steps + q);
+}
+
 void set_step_counter(Session &s) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
 }
@@ -549,16 +782,15 @@ void enqueue_steps(Session &s, uint64_t count) {
         for (uint64_t done = 0; done < count;) {
             const uint64_t k = std::min<uint64_t>(count - done, 256);
             hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
-            for (uint64_t q = 0; q < k; q++) launch_step(s, uint32_t(q), s.steps + q);
+            launch_steps(s, k);
             s.steps += k;
-            s.pending_steps = k;
             collect_timing(s);
             done += k;
         }
         return;
     }
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
-    for (uint64_t q = 0; q < count; q++) launch_step(s, uint32_t(q), s.steps + q);
+    launch_steps(s, count);
     s.steps += count;
 }
 

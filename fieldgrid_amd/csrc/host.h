@@ -87,7 +87,7 @@ struct Session {
     bool timing = false;
     bool observe = true;  // false: steps skip the observers (a recompute that already has them)
     double kernel_ms[3] = {0, 0, 0};  // step kernel(s), fix, observe
-    uint64_t kernel_count = 0, pending_steps = 0;
+    uint64_t kernel_count = 0;
     // buffers
     DevBuf<float> E[2], H[2], psi[2];
     DevBuf<uint32_t> coef_index, step;
@@ -103,6 +103,13 @@ struct Session {
     DevBuf<uint32_t> patch;
     DevBuf<double> partial;
     DevBuf<MarchBlock> blk_int, blk_z, blk_pml;  // march blocks clear of CPML / touching z slabs only / any
+    // two steps per launch (march2): on when nothing but sources sits between two steps
+    bool fuse2 = false;
+    uint32_t tj2 = 12, seg2 = 32;
+    DevBuf<MarchBlock> blk2_int, blk2_z, blk2_pml;
+    DevBuf<uint32_t> src2_start;
+    DevBuf<uint4> src2_ent;
+    std::vector<std::pair<size_t, uint32_t>> units;  // timed launch units: (event slot, steps)
     std::vector<hipEvent_t> events;
     // host copies the batched sweep and reset() need
     uint32_t pml = 0, nmax = 0;
@@ -136,6 +143,8 @@ void reset(Session &s);
 // The fused march's blocks of TJ rows x `seg` planes, split by CPML reach: out[0] clear of
 // every slab, out[1] touching only z (k) slabs, out[2] touching an x or y slab.
 void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]);
+// The same for the two-step march (60 lanes x tj rows owned, two-cell halo, two-plane lag).
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]);
 
 // Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute a block
 // list so that XCD x receives a contiguous run of the logical order (k tiles fastest, then j

@@ -217,3 +217,25 @@ def test_debye_boxes_match_the_oracle(gpu, per_box_tau):
     o32 = O.run_cpu(sim.desc, 0, W.boxes_pulse(), ex, 32)
     assert sim.problem().n_debye > 0
     _exact(rec, o32, f"boxes {'per-box' if per_box_tau else 'global'}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seg2", ["32", "5", "1"])
+def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2):
+    """march2 (two steps per pass, the intermediate step in registers) against one step per
+    pass and against the FP32 oracle: every output bit-identical. Odd grid sizes, an odd
+    check_every (an odd step count per chunk ends on a single step), segments of 1, 5 and 32
+    planes (segment starts at plane 0, 1 and further in), CPML 6, two ports, E and H probes."""
+    sim = W.layered(dims=(70, 90, 66), dx=1e-3, pml=6, port_i=(20, 41))
+    pulse = W.layered_pulse()
+    ex = F.Extras(max_steps=301, decay_db=math.inf, check_every=77,
+                  probes=[(2, 30, 40, 33), (4, 50, 52, 20), (0, 35, 45, 10)])
+    monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    two = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    monkeypatch.setenv("FIELDGRID_FUSE2", "0")
+    one = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    for name in ("series", "probes", "e", "h"):
+        a, b = getattr(two, name), getattr(one, name)
+        print(f"  {name}: {int(np.sum(a != b))} of {a.size} differ")
+        assert np.array_equal(a, b), name
+    _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"layered seg2 {seg2}")
