@@ -291,10 +291,12 @@ def main():
     # algorithmic bytes of the dominant (sweep) kernel per launch: E/H/coef/psi traffic only
     core_bpn = W.bytes_per_node_step(sim, None, core=True)
     achieved = core_bpn * nn / (sweep_ms * 1e-3) / 1e9
-    # HBM traffic per sweep from the committed rocprofv3 PMC summary of this workload
+    # two steps per launch (march2) when nothing but port sources acts between steps
+    two = args.engine == "fused" and os.environ.get("FIELDGRID_FUSE2", "1") != "0" and args.workload in ("layered", "plain")
+    # HBM traffic per step of the sweep from the committed rocprofv3 PMC summary of this workload
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"k_march_{args.workload}_traffic.json")
-    if args.engine == "fused" and os.path.exists(tfile):
+    if args.engine == "fused" and two == (args.workload == "layered") and os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("traffic_bytes_per_step")
     line = {
@@ -308,7 +310,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": round(core_bpn * nn),
-                     "kernel": ("k_march sweep (interior, z-CPML, CPML launches)" if args.engine == "fused"
+                     "kernel": ("k_march2 sweep, two steps per launch (interior, z-CPML, CPML launches); per step"
+                                if two else "k_march sweep (interior, z-CPML, CPML launches)" if args.engine == "fused"
                                 else "k_update_h + k_update_e"),
                      "kernel_ms": round(sweep_ms, 4), "alg_bytes_per_node": round(core_bpn, 3),
                      "fix_ms": round(ms[1] / max(timed, 1), 4), "observe_ms": round(ms[2] / max(timed, 1), 4)},
