@@ -238,6 +238,273 @@ __global__ __launch_bounds__(256) void k_adj_h(Geo g, float *__restrict__ lamE, 
     adj_h_node(g, lamE, la, lamH, lp, ln, in, i, j, k, t, y);
 }
 
+// ---- fused adjoint march: one pass per adjoint step ----------------------------------------
+//
+// The transposed step has the forward step's shape with the roles reversed: lamH gathers the
+// hv/hu terms of lamE at the node and its +1 neighbours (as H gathers E), then lamE gathers the
+// ev/eu terms of the new lamH at the node and its -1 neighbours (as E gathers H). So it marches
+// like k_march: a block of TJ+2 rows x 64 lanes (rows 1..TJ and lanes 1..62 owned) walks a
+// segment of i-planes; the +x terms are computed one plane ahead, the -x terms carried from the
+// plane before, +/-z by lane shuffles, +/-y through LDS, one barrier per plane. lamE, lamH and
+// the psi adjoints are ping-ponged (halo rows recompute from the old copies). The terms are the
+// expressions of e_terms / h_terms / adj_h_node, so the result equals k_adj_e + k_adj_h bit for bit.
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchBlock *__restrict__ blocks,
+                                                             const float *__restrict__ Lo, float *__restrict__ Ln,
+                                                             const float *__restrict__ Mo, float *__restrict__ Mn,
+                                                             const float *__restrict__ po, float *__restrict__ pn,
+                                                             uint32_t seg, GradArgs ga, InjectArgs in) {
+    constexpr int ROWS = TJ + 2;
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    __shared__ float xv[2][ROWS + 2][2][64];  // hv of Ex, hu of Ez at a plane, per row (read by the row above)
+    __shared__ float xw[2][ROWS + 2][2][64];  // ev of Hx, eu of Hz at a plane, per row (read by the row below)
+    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
+    __shared__ float xa[SEG_MAX + 2][PXY ? 14 : 4];
+    const MarchBlock mb = blocks[blockIdx.x];
+    if (mb.bx == ~0u) return;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k = int(mb.bx) * OWN_K + lane - 1;
+    const int j = int(mb.by) * TJ + w - 1;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 1 && lane <= OWN_K && w >= 1 && w <= TJ;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    const uint32_t ob4 = off * 4u;
+    const uint32_t ldA = act ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
+    if constexpr (PML) {
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
+    }
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    const uint32_t istart = i0 > 0 ? i0 - 1 : 0;
+    for (uint32_t t = w * 64 + lane; t + istart <= i1; t += 64 * ROWS) {
+        const uint32_t q = min(istart + t, g.n0 - 1);
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        xa[t][2] = __int_as_float(g.ax[0].slot_h[q]), xa[t][3] = __int_as_float(g.ax[0].slot_e[q]);
+        if constexpr (PXY) {
+            xa[t][4] = g.ax[0].bh[q], xa[t][5] = g.ax[0].ch[q], xa[t][6] = g.ax[0].raw_d[q];
+            xa[t][7] = g.ax[0].be[q], xa[t][8] = g.ax[0].ce[q], xa[t][9] = g.ax[0].raw_dd[q];
+        }
+    }
+    if (w < 2)
+        for (int q = 0; q < 2; q++) {
+            xv[q][w * (ROWS + 1)][0][lane] = 0.f, xv[q][w * (ROWS + 1)][1][lane] = 0.f;
+            xw[q][w * (ROWS + 1)][0][lane] = 0.f, xw[q][w * (ROWS + 1)][1][lane] = 0.f;
+        }
+    __syncthreads();  // sets and xa (slot numbers become buffer bases) are read from here on
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    // the hv/hu terms of plane p from lamE, (ca, cb) and the E-psi adjoints of that plane
+    struct Ed {
+        float lx, ly, lz;                          // lamE
+        float2 f0, f1, f2;                         // (ca, cb)
+        float pxy, pxz, pyz, pyx, pzx, pzy;        // E-psi adjoints (component, axis)
+        float hv0, hu0, hv1, hu1, hv2, hu2;        // terms
+    };
+    auto load_e = [&](uint32_t p, Ed &d) {
+        const bool in_ = p < g.n0;
+        const float *q = Lo + (in_ ? uint64_t(p) * plane : 0);
+        const uint32_t b = in_ ? pl4 : 0u;
+        d.lx = ld(q, ldA, b), d.ly = ld(q + nn, ldA, b), d.lz = ld(q + 2 * nn, ldA, b);
+        const uint64_t cb_ = in_ ? uint64_t(p) * plane : 0;
+        const uint32_t cbytes = in_ ? plc : 0u;
+        d.f0 = coef(cidx(cb_, cbytes)), d.f1 = coef(cidx(nn + cb_, cbytes)), d.f2 = coef(cidx(2 * nn + cb_, cbytes));
+        d.pxy = d.pxz = d.pyz = d.pyx = d.pzx = d.pzy = 0.f;
+        if constexpr (PZ) {
+            const uint64_t zrow = in_ ? uint64_t(p) * g.n1 * p2 : 0;
+            const uint32_t zb = in_ ? zbytes : 0u;
+            d.pxz = ld(po + PSI(0, 0, 2) + zrow, oze, zb), d.pyz = ld(po + PSI(0, 1, 2) + zrow, oze, zb);
+        }
+        if constexpr (PXY) {
+            const float *xs = xa[min(p, i1) - istart];
+            const int sex = __float_as_int(xs[3]);
+            const uint64_t yrow = in_ ? uint64_t(p) * p2 * g.n2 : 0, xeb = uint64_t(max(sex, 0)) * plane;
+            const uint32_t yb = in_ ? ybytes : 0u, xb = in_ ? pl4 : 0u, oxe = sex >= 0 ? ldA : OOB;
+            d.pxy = ld(po + PSI(0, 0, 1) + yrow, oye, yb);
+            d.pyx = ld(po + PSI(0, 1, 0) + xeb, oxe, xb);
+            d.pzx = ld(po + PSI(0, 2, 0) + xeb, oxe, xb), d.pzy = ld(po + PSI(0, 2, 1) + yrow, oye, yb);
+        }
+    };
+    auto terms_e = [&](uint32_t p, Ed &d) {
+        const float *xs = xa[min(p, i1) - istart];
+        const float iddx = xs[1];
+        const float ax_ = d.f0.y != 0.f ? d.f0.y * d.lx : 0.f, ay_ = d.f1.y != 0.f ? d.f1.y * d.ly : 0.f,
+                    az_ = d.f2.y != 0.f ? d.f2.y * d.lz : 0.f;
+        d.hv0 = ax_ * iddy, d.hu0 = -ax_ * iddz;
+        d.hv1 = ay_ * iddz, d.hu1 = -ay_ * iddx;
+        d.hv2 = az_ * iddx, d.hu2 = -az_ * iddy;
+        if constexpr (PZ) {
+            d.hu0 += cez * rddz * (d.pxz - ax_);
+            d.hv1 += cez * rddz * (d.pyz + ay_);
+        }
+        if constexpr (PXY) {
+            const float cex = xs[8], rddx = xs[9];
+            d.hv0 += cey * rddy * (d.pxy + ax_);
+            d.hu1 += cex * rddx * (d.pyx - ay_);
+            d.hv2 += cex * rddx * (d.pzx + az_);
+            d.hu2 += cey * rddy * (d.pzy - az_);
+        }
+    };
+    Ed cur, nxt;
+    load_e(istart, cur);
+    terms_e(istart, cur);
+    xv[istart & 1][w + 1][0][lane] = cur.hv0, xv[istart & 1][w + 1][1][lane] = cur.hu2;
+    __syncthreads();
+    float evz_p = 0.f, euy_p = 0.f;  // ev of Hz, eu of Hy at the plane before (the -x terms)
+    for (uint32_t i = istart; i < i1; i++) {
+        const int b = i & 1;
+        const uint64_t pb = uint64_t(i) * plane;
+        const bool il = i + 1 < g.n0;
+        const bool wplane = i >= i0;
+        const uint32_t sto = wplane ? stO : OOB;
+        const float *xs = xa[i - istart];
+        const float idx_ = xs[0];
+        load_e(i + 1, nxt);
+        const float hx0 = ld(Mo + pb, ldA, pl4), hy0 = ld(Mo + pb + nn, ldA, pl4), hz0 = ld(Mo + pb + 2 * nn, ldA, pl4);
+        float qxy = 0, qxz = 0, qyz = 0, qyx = 0, qzx = 0, qzy = 0;  // H-psi adjoints of plane i
+        uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
+        uint32_t oxh = OOB, oxe = OOB;
+        float bhx = 0, chx = 0, rdx = 0, bex = 0;
+        if constexpr (PZ) {
+            zrow = uint64_t(i) * g.n1 * p2;
+            qxz = ld(po + PSI(1, 0, 2) + zrow, ozh, zbytes), qyz = ld(po + PSI(1, 1, 2) + zrow, ozh, zbytes);
+        }
+        if constexpr (PXY) {
+            const int shx = __float_as_int(xs[2]), sex = __float_as_int(xs[3]);
+            bhx = xs[4], chx = xs[5], rdx = xs[6], bex = xs[7];
+            yrow = uint64_t(i) * p2 * g.n2;
+            xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
+            oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
+            qxy = ld(po + PSI(1, 0, 1) + yrow, oyh, ybytes);
+            qyx = ld(po + PSI(1, 1, 0) + xhb, oxh, pl4);
+            qzx = ld(po + PSI(1, 2, 0) + xhb, oxh, pl4), qzy = ld(po + PSI(1, 2, 1) + yrow, oyh, ybytes);
+        }
+        terms_e(i + 1, nxt);
+        // lamH: + terms at the node, - terms at +z (next lane), +y (row below, LDS), +x (next plane)
+        const float x1z = __shfl_down(cur.hv1, 1, 64), x0z = __shfl_down(cur.hu0, 1, 64);
+        const float x0y = xv[b][w + 2][0][lane], x2y = xv[b][w + 2][1][lane];
+        const float mx = hx0 + ((cur.hv1 - x1z) + (cur.hu2 - x2y));
+        const float my = hy0 + ((cur.hv2 - nxt.hv2) + (cur.hu0 - x0z));
+        const float mz = hz0 + ((cur.hv0 - x0y) + (cur.hu1 - nxt.hu1));
+        {
+            float *q = Mn + pb;
+            st(q, sto, pl4, mx), st(q + nn, sto, pl4, my), st(q + 2 * nn, sto, pl4, mz);
+        }
+        // ev/eu terms of the new lamH: zero outside the grid (the halo's out-of-grid nodes hold
+        // partial sums) and for the H components the forward step skips
+        const float bx_ = act && jl && kl ? -g.k_mu * mx : 0.f, by_ = act && kl && il ? -g.k_mu * my : 0.f,
+                    bz_ = act && il && jl ? -g.k_mu * mz : 0.f;
+        float ev0 = bx_ * idy, eu0 = -bx_ * idz, ev1 = by_ * idz, eu1 = -by_ * idx_, ev2 = bz_ * idx_, eu2 = -bz_ * idy;
+        if constexpr (PZ) {
+            eu0 += chz * rdz * (qxz - bx_);
+            ev1 += chz * rdz * (qyz + by_);
+        }
+        if constexpr (PXY) {
+            ev0 += chy * rdy * (qxy + bx_);
+            eu1 += chx * rdx * (qyx - by_);
+            ev2 += chx * rdx * (qzx + bz_);
+            eu2 += chy * rdy * (qzy - bz_);
+        }
+        xw[b][w + 1][0][lane] = ev0, xw[b][w + 1][1][lane] = eu2;
+        xv[b ^ 1][w + 1][0][lane] = nxt.hv0, xv[b ^ 1][w + 1][1][lane] = nxt.hu2;
+        __syncthreads();
+        // lamE: ca lamE + terms at -z (previous lane), -y (row above, LDS), -x (plane before)
+        const float y1z = __shfl_up(ev1, 1, 64), y0z = __shfl_up(eu0, 1, 64);
+        const float y0y = xw[b][w][0][lane], y2y = xw[b][w][1][lane];
+        float le0 = cur.f0.y != 0.f ? cur.lx * cur.f0.x : cur.lx;
+        float le1 = cur.f1.y != 0.f ? cur.ly * cur.f1.x : cur.ly;
+        float le2 = cur.f2.y != 0.f ? cur.lz * cur.f2.x : cur.lz;
+        le0 += (y1z - ev1) + (y2y - eu2);
+        le1 += (evz_p - ev2) + (y0z - eu0);
+        le2 += (y0y - ev0) + (euy_p - eu1);
+        if (own && wplane) {
+            const uint64_t id = pb + off;
+            if (in.ds)
+                for (uint32_t q = 0; q < in.n_obs; q++) {
+                    const uint64_t f = in.obs[q];
+                    if (f == id) le0 += float(in.ds[q]);
+                    if (f == nn + id) le1 += float(in.ds[q]);
+                    if (f == 2 * nn + id) le2 += float(in.ds[q]);
+                }
+            // gradient terms of the design edges at this node, with the lamE the step started from
+            const uint32_t bi = i - ga.lo[0], bj = uj - ga.lo[1], bk = uk - ga.lo[2];
+            if (bi < ga.nb[0] && bj < ga.nb[1] && bk < ga.nb[2]) {
+                const uint64_t bq = (uint64_t(bi) * ga.nb[1] + bj) * ga.nb[2] + bk,
+                               nb = uint64_t(ga.nb[0]) * ga.nb[1] * ga.nb[2];
+                const float lv[3] = {cur.lx, cur.ly, cur.lz};
+                for (int c = 0; c < 3; c++) {
+                    const int32_t e = ga.map[c * nb + bq];
+                    if (e < 0) continue;
+                    const double l = lv[c];
+                    const float2 tp = ga.tape[e];
+                    ga.gca[e] += l * double(tp.x);
+                    ga.gcb[e] += l * double(tp.y);
+                }
+            }
+        }
+        {
+            float *q = Ln + pb;
+            st(q, sto, pl4, le0), st(q + nn, sto, pl4, le1), st(q + 2 * nn, sto, pl4, le2);
+        }
+        // psi adjoints of this plane into the other copy: advanced where the forward advanced them
+        if constexpr (PZ) {
+            const float ax_ = cur.f0.y * cur.lx, ay_ = cur.f1.y * cur.ly;
+            const bool on0 = cur.f0.y != 0.f, on1 = cur.f1.y != 0.f;
+            st(pn + PSI(0, 0, 2) + zrow, sto == OOB ? OOB : oze, zbytes, on0 ? bez * (cur.pxz - ax_) : cur.pxz);
+            st(pn + PSI(0, 1, 2) + zrow, sto == OOB ? OOB : oze, zbytes, on1 ? bez * (cur.pyz + ay_) : cur.pyz);
+            st(pn + PSI(1, 0, 2) + zrow, sto == OOB ? OOB : ozh, zbytes, jl && kl ? bhz * (qxz - bx_) : qxz);
+            st(pn + PSI(1, 1, 2) + zrow, sto == OOB ? OOB : ozh, zbytes, kl && il ? bhz * (qyz + by_) : qyz);
+        }
+        if constexpr (PXY) {
+            const float ax_ = cur.f0.y * cur.lx, ay_ = cur.f1.y * cur.ly, az_ = cur.f2.y * cur.lz;
+            const bool on0 = cur.f0.y != 0.f, on1 = cur.f1.y != 0.f, on2 = cur.f2.y != 0.f;
+            const uint32_t ye = sto == OOB ? OOB : oye, yh = sto == OOB ? OOB : oyh;
+            const uint32_t xe_ = sto == OOB ? OOB : oxe, xh_ = sto == OOB ? OOB : oxh;
+            st(pn + PSI(0, 0, 1) + yrow, ye, ybytes, on0 ? bey * (cur.pxy + ax_) : cur.pxy);
+            st(pn + PSI(0, 1, 0) + xeb, xe_, pl4, on1 ? bex * (cur.pyx - ay_) : cur.pyx);
+            st(pn + PSI(0, 2, 0) + xeb, xe_, pl4, on2 ? bex * (cur.pzx + az_) : cur.pzx);
+            st(pn + PSI(0, 2, 1) + yrow, ye, ybytes, on2 ? bey * (cur.pzy - az_) : cur.pzy);
+            st(pn + PSI(1, 0, 1) + yrow, yh, ybytes, jl && kl ? bhy * (qxy + bx_) : qxy);
+            st(pn + PSI(1, 1, 0) + xhb, xh_, pl4, kl && il ? bhx * (qyx - by_) : qyx);
+            st(pn + PSI(1, 2, 0) + xhb, xh_, pl4, il && jl ? bhx * (qzx + bz_) : qzx);
+            st(pn + PSI(1, 2, 1) + yrow, yh, ybytes, il && jl ? bhy * (qzy - bz_) : qzy);
+        }
+        evz_p = ev2, euy_p = eu1;
+        cur = nxt;
+    }
+#undef PSI
+}
+
 // dJ/ds_n enters lamE at the observed edges (s_n is E after step n)
 __global__ void k_adj_inject(Geo g, float *__restrict__ lamE, float *__restrict__ la,
                              const uint64_t *__restrict__ obs, uint32_t n_obs, const double *__restrict__ ds) {
@@ -245,7 +512,7 @@ __global__ void k_adj_inject(Geo g, float *__restrict__ lamE, float *__restrict_
     for (uint32_t q = 0; q < n_obs; q++) {  // one thread: observations may share an edge
         const uint64_t f = obs[q];
         lamE[f] += float(ds[q]);
-        la[f] = coef_at(g, f).y * lamE[f];
+        if (la) la[f] = coef_at(g, f).y * lamE[f];
     }
 }
 
@@ -270,6 +537,46 @@ __global__ __launch_bounds__(256) void k_adj_tape(Geo g, const uint64_t *__restr
         curl = (Hn[nn + id] - Hn[nn + id - sx]) * g.ax[0].inv_dd[i] - (Hn[id] - Hn[id - sy]) * g.ax[1].inv_dd[j];
     }
     tape[e] = make_float2(Eo[f], curl);
+}
+
+// The tape of both steps of a two-step launch: step n's E^n is in the copy the launch read (Eo)
+// and its curl^n comes from H^{n+1/2} recomputed from that copy; step n+1's E^{n+1} is
+// recomputed from it the same way, and its curl^{n+1} comes from the new H copy (Hn).
+__global__ __launch_bounds__(256) void k_adj_tape2(Geo g, FixArgs f, const uint64_t *__restrict__ edges, uint32_t ne,
+                                                   const float *__restrict__ Eo, const float *__restrict__ Ho,
+                                                   const float *__restrict__ Po, const float *__restrict__ Hn,
+                                                   uint32_t s_off, float2 *__restrict__ tape0, float2 *__restrict__ tape1) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const uint64_t fl = edges[e], nn = g.nn;
+    const uint32_t c = uint32_t(fl / nn);
+    const uint64_t id = fl - c * nn;
+    const uint32_t k = uint32_t(id % g.n2), j = uint32_t((id / g.n2) % g.n1), i = uint32_t(id / (uint64_t(g.n1) * g.n2));
+    const uint32_t n = f.step[0] + s_off;
+    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
+    float curl0;
+    if (c == 0) {  // u = y, v = z: hv = Hz(p)-Hz(p-y), hu = Hy(p)-Hy(p-z)
+        curl0 = (h_mid<2>(g, Eo, Ho, Po, i, j, k) - h_mid<2>(g, Eo, Ho, Po, i, j - 1, k)) * g.ax[1].inv_dd[j] -
+                (h_mid<1>(g, Eo, Ho, Po, i, j, k) - h_mid<1>(g, Eo, Ho, Po, i, j, k - 1)) * g.ax[2].inv_dd[k];
+    } else if (c == 1) {  // u = z, v = x: hv = Hx(p)-Hx(p-z), hu = Hz(p)-Hz(p-x)
+        curl0 = (h_mid<0>(g, Eo, Ho, Po, i, j, k) - h_mid<0>(g, Eo, Ho, Po, i, j, k - 1)) * g.ax[2].inv_dd[k] -
+                (h_mid<2>(g, Eo, Ho, Po, i, j, k) - h_mid<2>(g, Eo, Ho, Po, i - 1, j, k)) * g.ax[0].inv_dd[i];
+    } else {  // u = x, v = y: hv = Hy(p)-Hy(p-x), hu = Hx(p)-Hx(p-y)
+        curl0 = (h_mid<1>(g, Eo, Ho, Po, i, j, k) - h_mid<1>(g, Eo, Ho, Po, i - 1, j, k)) * g.ax[0].inv_dd[i] -
+                (h_mid<0>(g, Eo, Ho, Po, i, j, k) - h_mid<0>(g, Eo, Ho, Po, i, j - 1, k)) * g.ax[1].inv_dd[j];
+    }
+    tape0[e] = make_float2(Eo[fl], curl0);
+    const float e1 = field_mid(g, f, pl, Eo, Ho, Po, fl);
+    const uint64_t sx = uint64_t(g.n1) * g.n2, sy = g.n2;
+    float curl1;
+    if (c == 0) {
+        curl1 = (Hn[2 * nn + id] - Hn[2 * nn + id - sy]) * g.ax[1].inv_dd[j] - (Hn[nn + id] - Hn[nn + id - 1]) * g.ax[2].inv_dd[k];
+    } else if (c == 1) {
+        curl1 = (Hn[id] - Hn[id - 1]) * g.ax[2].inv_dd[k] - (Hn[2 * nn + id] - Hn[2 * nn + id - sx]) * g.ax[0].inv_dd[i];
+    } else {
+        curl1 = (Hn[nn + id] - Hn[nn + id - sx]) * g.ax[0].inv_dd[i] - (Hn[id] - Hn[id - sy]) * g.ax[1].inv_dd[j];
+    }
+    tape1[e] = make_float2(e1, curl1);
 }
 
 // ---- f64 forward (finite-difference checks) -------------------------------------------------
@@ -390,7 +697,7 @@ struct fg_adjoint {
     std::unique_ptr<fg::Session> s;
     std::vector<fg::DesignEdge> edges;
     bool built = false, have_forward = false;
-    fg::DevBuf<float> snaps, lamE, lamA, lamH, lamP[2];
+    fg::DevBuf<float> snaps, lamE, lamA, lamH, lamH2, lamP[2];
     fg::DevBuf<uint64_t> d_edges, d_obs;
     fg::DevBuf<int32_t> dmap;
     fg::DevBuf<float2> tape;
@@ -489,6 +796,36 @@ static void forward(fg_adjoint &A, float *series) {
     A.have_forward = true;
 }
 
+// One adjoint step through k_adj_march over the session's march block lists (the same reach as
+// the forward march: one halo row/lane each side and one warm-up plane).
+static void adj_march(Session &s, const float *Lo, float *Ln, const float *Mo, float *Mn, const float *po, float *pn,
+                      const GradArgs &ga, const InjectArgs &in) {
+    hipStream_t st = s.stream;
+    const Geo &g = s.geo;
+    const dim3 block(64, s.tj + 2);
+    const uint32_t ni = uint32_t(s.blk_int.n), nz = uint32_t(s.blk_z.n), np_ = uint32_t(s.blk_pml.n);
+    const MarchBlock *bi = s.blk_int.p, *bz = s.blk_z.p, *bp = s.blk_pml.p;
+#define FG_AM2(TJ, W, L)                                                                                                     \
+    do {                                                                                                                     \
+        if (ni) hipLaunchKernelGGL((k_adj_march<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+        if (nz) hipLaunchKernelGGL((k_adj_march<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+        if (np_) hipLaunchKernelGGL((k_adj_march<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+    } while (0)
+#define FG_AM(TJ)                                                                     \
+    do {                                                                              \
+        const bool lds = g.n_sets <= uint32_t(SETS_LDS);                              \
+        if (g.wide) {                                                                 \
+            if (lds) FG_AM2(TJ, true, true); else FG_AM2(TJ, true, false);             \
+        } else {                                                                      \
+            if (lds) FG_AM2(TJ, false, true); else FG_AM2(TJ, false, false);           \
+        }                                                                             \
+    } while (0)
+    if (s.tj == 8) FG_AM(8);
+    else FG_AM(14);
+#undef FG_AM
+#undef FG_AM2
+}
+
 static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     if (!A.have_forward) throw HipError{"fg_adjoint_backward needs a forward run first", FG_INVALID};
     Session &s = *A.s;
@@ -499,6 +836,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         A.lamE.alloc(3 * nn);
         A.lamA.alloc(3 * nn);
         A.lamH.alloc(3 * nn);
+        A.lamH2.alloc(3 * nn);
         A.lamP[0].alloc(std::max<uint64_t>(psi_n, 1));
         A.lamP[1].alloc(std::max<uint64_t>(psi_n, 1));
         std::vector<uint64_t> f(ne), o(n_obs);
@@ -523,7 +861,13 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         }
         A.dmap.upload(map.data(), map.size(), st);
     }
-    A.lamE.zero(st), A.lamA.zero(st), A.lamH.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
+    A.lamE.zero(st), A.lamA.zero(st), A.lamH.zero(st), A.lamH2.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
+    // the fused adjoint march (one pass per adjoint step) unless FIELDGRID_ADJ_MARCH=0 asks for
+    // the two gather kernels; the march ping-pongs lamE (lamE/lamA) and lamH (lamH/lamH2)
+    const char *am = getenv("FIELDGRID_ADJ_MARCH");
+    const bool march = s.fused && !(am && am[0] == '0');
+    float *LE[2] = {A.lamE.p, A.lamA.p}, *LH[2] = {A.lamH.p, A.lamH2.p};
+    int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
     A.dseries.upload(dseries, A.steps * n_obs, st);
     std::vector<hipEvent_t> ev(3 * A.n_snaps);
@@ -538,8 +882,8 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     const auto t0 = std::chrono::steady_clock::now();
     // dJ/ds of the last step; every earlier one enters at the end of the next step's transpose
     if (n_obs && A.steps)
-        hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, st, g, A.lamE.p, A.lamA.p, A.d_obs.p, uint32_t(n_obs),
-                           A.dseries.p + (A.steps - 1) * n_obs);
+        hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, st, g, A.lamE.p, march ? nullptr : A.lamA.p, A.d_obs.p,
+                           uint32_t(n_obs), A.dseries.p + (A.steps - 1) * n_obs);
     for (uint64_t k = A.n_snaps; k-- > 0;) {
         const uint64_t start = k * A.every, end = std::min(start + A.every, A.steps);
         FG_CHECK(hipEventRecord(ev[3 * k], st));
@@ -556,12 +900,22 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         s.steps = start;
         s.observe = false;  // the series of this segment is already known
         set_step_counter(s);
-        for (uint64_t n = start; n < end; n++) {
+        for (uint64_t n = start; n < end;) {
             const int prev = s.cur;
-            launch_step(s, uint32_t(n - start), n);
-            if (ne)
-                hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
-                                   s.H[s.cur].p, A.tape.p + (n - start) * ne);
+            if (s.fuse2 && n + 2 <= end) {  // two steps per launch, both taped
+                launch_pair(s, uint32_t(n - start));
+                if (ne)
+                    hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
+                                       s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
+                                       A.tape.p + (n - start) * ne, A.tape.p + (n + 1 - start) * ne);
+                n += 2;
+            } else {
+                launch_step(s, uint32_t(n - start), n);
+                if (ne)
+                    hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
+                                       s.H[s.cur].p, A.tape.p + (n - start) * ne);
+                n += 1;
+            }
         }
         s.steps = end;
         s.observe = true;
@@ -569,9 +923,14 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         for (uint64_t n = end; n-- > start;) {
             ga.tape = A.tape.p + (n - start) * ne;
             in.ds = n ? A.dseries.p + (n - 1) * n_obs : nullptr;
-            hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
-            hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamA.p, A.lamH.p, A.lamP[pin].p,
-                               A.lamP[pin ^ 1].p, in);
+            if (march) {
+                adj_march(s, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga, in);
+                le ^= 1;
+            } else {
+                hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
+                hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamA.p, A.lamH.p, A.lamP[pin].p,
+                                   A.lamP[pin ^ 1].p, in);
+            }
             pin ^= 1;
         }
         FG_CHECK(hipEventRecord(ev[3 * k + 2], st));

@@ -1021,4 +1021,59 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
 #undef PSI
 }
 
+// The first step of a two-step launch exists only inside it. What reads it (the observers of
+// that step, the adjoint's tape) recomputes the few values it needs from the untouched previous
+// copy, with the march's arithmetic: H by h_next, E by the same update, then the driven port's
+// sources.
+template <int C>
+__device__ float h_mid(const Geo &g, const float *__restrict__ E, const float *__restrict__ H,
+                       const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
+    return h_next<C>(g, E, H, psi, nullptr, i, j, k);
+}
+
+template <int C>
+__device__ float e_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
+                       const float *__restrict__ H, const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
+    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
+    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
+    const float eold = E[C * g.nn + id];
+    const float2 cf = coef_at(g, C * g.nn + id);
+    float e = eold;
+    if (cf.y != 0.f) {
+        uint32_t um[3] = {i, j, k}, vm[3] = {i, j, k};
+        um[U] -= 1;
+        vm[V] -= 1;
+        const float hv = h_mid<V>(g, E, H, psi, i, j, k) - h_mid<V>(g, E, H, psi, um[0], um[1], um[2]);
+        const float hu = h_mid<U>(g, E, H, psi, i, j, k) - h_mid<U>(g, E, H, psi, vm[0], vm[1], vm[2]);
+        const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
+        float t1 = hv * g.ax[U].inv_dd[pu], t2 = hu * g.ax[V].inv_dd[pv];
+        if (g.pml) {
+            const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
+            if (su >= 0)
+                t1 += g.ax[U].be[pu] * psi[psi_at<U>(g, 0, C, i, j, k, uint32_t(su))] + g.ax[U].ce[pu] * hv * g.ax[U].raw_dd[pu];
+            if (sv >= 0)
+                t2 += g.ax[V].be[pv] * psi[psi_at<V>(g, 0, C, i, j, k, uint32_t(sv))] + g.ax[V].ce[pv] * hu * g.ax[V].raw_dd[pv];
+        }
+        e = cf.x * eold + cf.y * (t1 - t2);
+    }
+    for (uint32_t q = 0; q < f.n_src; q++)
+        if (f.src[q].comp == C && f.src[q].id == id) e -= f.src[q].src * pl;
+    return e;
+}
+
+__device__ inline float field_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
+                           const float *__restrict__ H, const float *__restrict__ psi, uint64_t flat) {
+    const uint32_t c = uint32_t(flat / g.nn);
+    const uint64_t id = flat - c * g.nn;
+    const uint32_t k = uint32_t(id % g.n2), j = uint32_t((id / g.n2) % g.n1), i = uint32_t(id / (uint64_t(g.n1) * g.n2));
+    switch (c) {
+        case 0: return e_mid<0>(g, f, pl, E, H, psi, i, j, k);
+        case 1: return e_mid<1>(g, f, pl, E, H, psi, i, j, k);
+        case 2: return e_mid<2>(g, f, pl, E, H, psi, i, j, k);
+        case 3: return h_mid<0>(g, E, H, psi, i, j, k);
+        case 4: return h_mid<1>(g, E, H, psi, i, j, k);
+        default: return h_mid<2>(g, E, H, psi, i, j, k);
+    }
+}
+
 }  // namespace fg

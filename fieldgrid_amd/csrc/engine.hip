@@ -111,59 +111,8 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
 }
 
 // The first step of a two-step launch exists only inside it. Its observers (port V/I, point
-// probes) recompute the few values they read from the untouched previous copy, with the
-// march's arithmetic: H by h_next, E by the same update, then the driven port's sources.
-template <int C>
-__device__ float h_mid(const Geo &g, const float *__restrict__ E, const float *__restrict__ H,
-                       const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
-    return h_next<C>(g, E, H, psi, nullptr, i, j, k);
-}
-
-template <int C>
-__device__ float e_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
-                       const float *__restrict__ H, const float *__restrict__ psi, uint32_t i, uint32_t j, uint32_t k) {
-    constexpr int U = (C + 1) % 3, V = (C + 2) % 3;
-    const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + k;
-    const float eold = E[C * g.nn + id];
-    const float2 cf = coef_at(g, C * g.nn + id);
-    float e = eold;
-    if (cf.y != 0.f) {
-        uint32_t um[3] = {i, j, k}, vm[3] = {i, j, k};
-        um[U] -= 1;
-        vm[V] -= 1;
-        const float hv = h_mid<V>(g, E, H, psi, i, j, k) - h_mid<V>(g, E, H, psi, um[0], um[1], um[2]);
-        const float hu = h_mid<U>(g, E, H, psi, i, j, k) - h_mid<U>(g, E, H, psi, vm[0], vm[1], vm[2]);
-        const uint32_t pu = coord<U>(i, j, k), pv = coord<V>(i, j, k);
-        float t1 = hv * g.ax[U].inv_dd[pu], t2 = hu * g.ax[V].inv_dd[pv];
-        if (g.pml) {
-            const int su = g.ax[U].slot_e[pu], sv = g.ax[V].slot_e[pv];
-            if (su >= 0)
-                t1 += g.ax[U].be[pu] * psi[psi_at<U>(g, 0, C, i, j, k, uint32_t(su))] + g.ax[U].ce[pu] * hv * g.ax[U].raw_dd[pu];
-            if (sv >= 0)
-                t2 += g.ax[V].be[pv] * psi[psi_at<V>(g, 0, C, i, j, k, uint32_t(sv))] + g.ax[V].ce[pv] * hu * g.ax[V].raw_dd[pv];
-        }
-        e = cf.x * eold + cf.y * (t1 - t2);
-    }
-    for (uint32_t q = 0; q < f.n_src; q++)
-        if (f.src[q].comp == C && f.src[q].id == id) e -= f.src[q].src * pl;
-    return e;
-}
-
-__device__ float field_mid(const Geo &g, const FixArgs &f, float pl, const float *__restrict__ E,
-                           const float *__restrict__ H, const float *__restrict__ psi, uint64_t flat) {
-    const uint32_t c = uint32_t(flat / g.nn);
-    const uint64_t id = flat - c * g.nn;
-    const uint32_t k = uint32_t(id % g.n2), j = uint32_t((id / g.n2) % g.n1), i = uint32_t(id / (uint64_t(g.n1) * g.n2));
-    switch (c) {
-        case 0: return e_mid<0>(g, f, pl, E, H, psi, i, j, k);
-        case 1: return e_mid<1>(g, f, pl, E, H, psi, i, j, k);
-        case 2: return e_mid<2>(g, f, pl, E, H, psi, i, j, k);
-        case 3: return h_mid<0>(g, E, H, psi, i, j, k);
-        case 4: return h_mid<1>(g, E, H, psi, i, j, k);
-        default: return h_mid<2>(g, E, H, psi, i, j, k);
-    }
-}
-
+// probes) recompute the few values they read from the untouched previous copy (h_mid, e_mid,
+// field_mid in device.h).
 __global__ __launch_bounds__(256) void k_observe_mid(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ E,
                                                      const float *__restrict__ H, const float *__restrict__ psi,
                                                      uint32_t s_off) {
@@ -721,7 +670,7 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
 // Enqueue two steps (s_off and s_off + 1) with one launch per block class of march2: the first
 // step's observers recompute from the copy the launch reads, the second's run as usual after
 // its sparse fixes.
-static void launch_pair(Session &s, uint32_t s_off) {
+void launch_pair(Session &s, uint32_t s_off) {
     hipStream_t st = s.stream;
     const bool timed = s.timing;
     const size_t slot = 4 * size_t(s_off);

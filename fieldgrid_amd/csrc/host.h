@@ -168,6 +168,8 @@ void enqueue_steps(Session &s, uint64_t count);
 // step at offset s_off from it (the caller advances s.steps afterwards).
 void set_step_counter(Session &s);
 void launch_step(Session &s, uint32_t s_off, uint64_t global_step);
+// Two steps (s_off, s_off + 1) in one march2 launch; only when s.fuse2.
+void launch_pair(Session &s, uint32_t s_off);
 
 // Sum of E^2 + (376.73 H)^2 over the latest fields (cpu.rs:417-421); synchronizes the stream.
 double energy(Session &s);
