@@ -41,6 +41,11 @@ struct BatchRun {
     float *E[2], *H[2], *psi[2];
     FixArgs fix;
     ObsArgs obs;
+    // two-step passes: the first step's sources and Debye ADE (march2), and the sparse fixes
+    // of the second step, which read the pole states and E the pass wrote
+    Src2 sr;
+    Deb2 db;
+    FixArgs fix2;
 };
 
 struct BatchBlock {
@@ -58,10 +63,34 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_march_batch(const BatchRun *_
                                    r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg);
 }
 
+constexpr int TJ2 = 12;  // rows per block of the two-step march
+
+template <int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun *__restrict__ runs,
+                                                                 const BatchBlock *__restrict__ blocks, uint32_t par,
+                                                                 uint32_t seg, uint32_t s_off) {
+    const BatchBlock bb = blocks[blockIdx.x];
+    if (bb.bx == ~0u) return;
+    const BatchRun &r = runs[bb.run];
+    const uint32_t n = r.fix.step[0] + s_off;
+    const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
+    march2<TJ2, PML, WIDE, LDS_SETS>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
+                                     r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db);
+}
+
 __global__ __launch_bounds__(256) void k_fix_batch(const BatchRun *__restrict__ runs, const uint32_t *__restrict__ list,
-                                                   uint32_t par, uint32_t s_off) {
+                                                   uint32_t par, uint32_t s_off, uint32_t second) {
     const BatchRun &r = runs[list[blockIdx.y]];
-    fix_step(r.g, r.fix, r.E[par ^ 1u], r.E[par], s_off, uint64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+    fix_step(r.g, second ? r.fix2 : r.fix, r.E[par ^ 1u], r.E[par], s_off,
+             uint64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// observers of the first step of a two-step pass, from the copy the pass read
+__global__ __launch_bounds__(256) void k_observe_mid_batch(const BatchRun *__restrict__ runs,
+                                                           const uint32_t *__restrict__ list, uint32_t par,
+                                                           uint32_t s_off) {
+    const BatchRun &r = runs[list[blockIdx.y]];
+    observe_mid(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], s_off, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_observe_batch(const BatchRun *__restrict__ runs,
@@ -112,8 +141,12 @@ struct Batch {
     std::vector<std::unique_ptr<Session>> runs;
     DevBuf<BatchRun> table;
     DevBuf<uint32_t> step;
-    DevBuf<BatchBlock> lists[N_LISTS];
-    uint32_t list_n[N_LISTS] = {};
+    DevBuf<BatchBlock> lists[N_LISTS], lists2[N_LISTS];  // one-step and two-step march blocks
+    uint32_t list_n[N_LISTS] = {}, list2_n[N_LISTS] = {};
+    // steps per march pass (1, or 2 when every run allows it) and the current ping-pong copy
+    uint32_t depth = 1, par = 0;
+    bool pairs_ok = false;
+    uint32_t seg2 = 32, seg2_base = 32;
     DevBuf<uint32_t> fix_list, obs_list;
     uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0;
     uint32_t tj = 14, seg = 32, chunk = 32, xcds = 8;
@@ -140,13 +173,27 @@ struct Batch {
 // Rebuild the merged block lists and the fix/observe run lists for the runs still going.
 static void plan(Batch &b, const std::vector<uint8_t> &active) {
     b.drop_graph();
-    std::vector<BatchBlock> L[N_LISTS];
+    std::vector<BatchBlock> L[N_LISTS], L2[N_LISTS];
     std::vector<MarchBlock> cls[3];
     std::vector<uint32_t> fl, ol;
     b.fix_gx = b.obs_gx = 0;
     // as runs finish, fewer blocks remain than the GPU holds: shorter segments (each block
-    // marches fewer planes) keep ~6 blocks per CU in flight. The results do not depend on it.
+    // marches fewer planes) keep ~6 blocks per CU in flight (3 for the two-step march, whose
+    // blocks are twice as long-lived). The results do not depend on it.
     b.seg = b.seg_base;
+    b.seg2 = b.seg2_base;
+    if (b.depth == 2)
+        for (;;) {
+            uint64_t blocks = 0;
+            for (uint32_t r = 0; r < b.runs.size(); r++)
+                if (active[r]) {
+                    const Session &s = *b.runs[r];
+                    blocks += uint64_t((s.n[2] + OWN_K2 - 1) / OWN_K2) * ((s.n[1] + TJ2 - 1) / TJ2) *
+                              ((s.n[0] + b.seg2 - 1) / b.seg2);
+                }
+            if (blocks >= 3ull * b.cus || b.seg2 <= 4) break;
+            b.seg2 /= 2;
+        }
     for (;;) {
         uint64_t blocks = 0;
         for (uint32_t r = 0; r < b.runs.size(); r++)
@@ -160,10 +207,16 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     for (uint32_t r = 0; r < b.runs.size(); r++) {
         if (!active[r]) continue;
         const Session &s = *b.runs[r];
-        march_classes(s, b.tj, b.seg, cls);
         const int key = (s.geo.wide ? 2 : 0) + (s.geo.n_sets <= uint32_t(SETS_LDS) ? 1 : 0);
+        // one-step lists always (a two-step batch still takes an odd last step alone)
+        march_classes(s, b.tj, b.seg, cls);
         for (int c = 0; c < 3; c++)
             for (const MarchBlock &m : cls[c]) L[c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
+        if (b.depth == 2) {
+            march_classes2(s, TJ2, b.seg2, cls);
+            for (int c = 0; c < 3; c++)
+                for (const MarchBlock &m : cls[c]) L2[c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
+        }
         if (s.fix_blocks) fl.push_back(r), b.fix_gx = std::max(b.fix_gx, s.fix_blocks);
         if (s.obs_blocks) ol.push_back(r), b.obs_gx = std::max(b.obs_gx, s.obs_blocks);
     }
@@ -171,6 +224,9 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
         xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
         b.lists[q].upload(L[q].data(), L[q].size(), b.stream);
         b.list_n[q] = uint32_t(L[q].size());
+        xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
+        b.lists2[q].upload(L2[q].data(), L2[q].size(), b.stream);
+        b.list2_n[q] = uint32_t(L2[q].size());
     }
     b.fix_list.upload(fl.data(), fl.size(), b.stream);
     b.obs_list.upload(ol.data(), ol.size(), b.stream);
@@ -183,6 +239,26 @@ template <int TJ, int PML, bool W, bool L>
 static void launch_list(Batch &b, int q, uint32_t par) {
     hipLaunchKernelGGL((k_march_batch<TJ, PML, W, L>), dim3(b.list_n[q]), dim3(64, TJ + 2), 0, b.stream, b.table.p,
                        b.lists[q].p, par, b.seg);
+}
+
+template <int PML, bool W, bool L>
+static void launch_list2(Batch &b, int q, uint32_t par, uint32_t s_off) {
+    hipLaunchKernelGGL((k_march2_batch<PML, W, L>), dim3(b.list2_n[q]), dim3(64, TJ2 + 4), 0, b.stream, b.table.p,
+                       b.lists2[q].p, par, b.seg2, s_off);
+}
+
+static void launch_march2(Batch &b, uint32_t par, uint32_t s_off) {
+    for (int q = 0; q < N_LISTS; q++) {
+        if (!b.list2_n[q]) continue;
+        switch (q) {
+#define FG_CASE(Q, PML, W, L) \
+    case Q: launch_list2<PML, W, L>(b, q, par, s_off); break;
+            FG_CASE(0, 0, false, false) FG_CASE(1, 0, false, true) FG_CASE(2, 0, true, false) FG_CASE(3, 0, true, true)
+            FG_CASE(4, 1, false, false) FG_CASE(5, 1, false, true) FG_CASE(6, 1, true, false) FG_CASE(7, 1, true, true)
+            FG_CASE(8, 2, false, false) FG_CASE(9, 2, false, true) FG_CASE(10, 2, true, false) FG_CASE(11, 2, true, true)
+#undef FG_CASE
+        }
+    }
 }
 
 template <int TJ>
@@ -206,29 +282,49 @@ static void launch_step(Batch &b, uint32_t s_off, uint32_t par) {
     else launch_march<14>(b, par);
     if (b.n_fix)
         hipLaunchKernelGGL(k_fix_batch, dim3(b.fix_gx, b.n_fix), dim3(256), 0, b.stream, b.table.p, b.fix_list.p, par,
-                           s_off);
+                           s_off, 0u);
     if (b.n_obs)
         hipLaunchKernelGGL(k_observe_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p, b.obs_list.p,
                            par, s_off);
 }
 
-// Capture `chunk` steps starting at an even step, ending with the counter advanced by chunk.
+// Two steps (s_off, s_off + 1) of every run still going in one pass of the two-step march.
+static void launch_pair(Batch &b, uint32_t s_off, uint32_t par) {
+    launch_march2(b, par, s_off);
+    if (b.n_fix)
+        hipLaunchKernelGGL(k_fix_batch, dim3(b.fix_gx, b.n_fix), dim3(256), 0, b.stream, b.table.p, b.fix_list.p, par,
+                           s_off + 1, 1u);
+    if (b.n_obs) {
+        hipLaunchKernelGGL(k_observe_mid_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p,
+                           b.obs_list.p, par, s_off);
+        hipLaunchKernelGGL(k_observe_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p, b.obs_list.p,
+                           par, s_off + 1);
+    }
+}
+
+// Capture `chunk` steps starting from copy 0, ending on copy 0 with the counter advanced by
+// chunk (an even number of passes: chunk is even for one step per pass, a multiple of 4 for two).
 static void capture(Batch &b) {
     b.drop_graph();
     FG_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeRelaxed));
-    for (uint32_t q = 0; q < b.chunk; q++) launch_step(b, q, q & 1u);
+    if (b.depth == 2)
+        for (uint32_t q = 0; q < b.chunk; q += 2) launch_pair(b, q, (q / 2) & 1u);
+    else
+        for (uint32_t q = 0; q < b.chunk; q++) launch_step(b, q, q & 1u);
     hipLaunchKernelGGL(k_batch_add_step, dim3(1), dim3(1), 0, b.stream, b.step.p, b.chunk);
     FG_CHECK(hipStreamEndCapture(b.stream, &b.graph));
     FG_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
 }
 
-// Enqueue `count` steps from step `from` (graph replays where a whole even-aligned chunk fits).
+// Enqueue `count` steps from step `from`: graph replays while on copy 0 and a whole chunk fits,
+// else direct passes (two steps each at depth 2, a single step for an odd remainder).
 static void advance(Batch &b, uint64_t from, uint64_t count, bool graphs) {
     uint64_t done = 0;
+    const bool replay = graphs && b.chunk >= 2;
     while (done < count) {
         const uint64_t at = from + done;
         hipLaunchKernelGGL(k_batch_set_step, dim3(1), dim3(1), 0, b.stream, b.step.p, uint32_t(at));
-        if (graphs && b.chunk >= 2 && (at & 1u) == 0 && count - done >= b.chunk) {
+        if (replay && b.par == 0 && count - done >= b.chunk) {
             if (!b.exec) capture(b);
             while (count - done >= b.chunk) {
                 FG_CHECK(hipGraphLaunch(b.exec, b.stream));
@@ -236,11 +332,23 @@ static void advance(Batch &b, uint64_t from, uint64_t count, bool graphs) {
             }
             continue;
         }
-        // direct launches: one step to reach an even step where replays can start, or the
-        // remainder shorter than a chunk
-        const uint64_t k = graphs && b.chunk >= 2 && (at & 1u) ? 1 : count - done;
-        for (uint64_t q = 0; q < k; q++) launch_step(b, uint32_t(q), uint32_t((at + q) & 1u));
-        done += k;
+        // direct passes: one to get back to copy 0 where replays can start, or the remainder
+        // shorter than a chunk
+        const uint64_t rem = count - done;
+        uint64_t q = 0;
+        const bool one = replay && b.par != 0 && rem >= b.chunk;
+        while (q < rem) {
+            if (b.depth == 2 && rem - q >= 2) {
+                launch_pair(b, uint32_t(q), b.par);
+                q += 2;
+            } else {
+                launch_step(b, uint32_t(q), b.par);
+                q += 1;
+            }
+            b.par ^= 1u;
+            if (one) break;
+        }
+        done += q;
     }
     FG_CHECK(hipGetLastError());
 }
@@ -329,36 +437,55 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
     const std::string key = shape_key(b);
     auto cache = read_cache(t ? t->cache : nullptr);
     const bool ask_tj = !t || !t->tj, ask_seg = !t || !t->seg, ask_chunk = !t || !t->chunk;
+    // step-fusion depth: two steps per pass only where every run allows it
+    const bool ask_depth = (!t || !t->depth) && b.pairs_ok;
+    b.depth = t && t->depth == 2 && b.pairs_ok ? 2 : 1;
     b.tj = t && t->tj ? (t->tj == 8 ? 8 : 14) : 14;
-    b.seg = b.seg_base = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
+    const uint32_t seg_in = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
+    b.seg = b.seg_base = b.seg2 = b.seg2_base = seg_in;
     b.chunk = t && t->chunk ? t->chunk : 32;
     bool from_cache = false, tuned = false;
     double step_ms = 0.0;
     auto it = cache.find(key);
-    if ((ask_tj || ask_seg || ask_chunk) && it != cache.end()) {
-        if (ask_tj) b.tj = uint32_t(it->second[0]) == 8 ? 8 : 14;
-        if (ask_seg) b.seg = b.seg_base = std::min<uint32_t>(std::max<uint32_t>(uint32_t(it->second[1]), 1), SEG_MAX);
-        if (ask_chunk) b.chunk = std::max<uint32_t>(uint32_t(it->second[2]), 1);
-        step_ms = it->second.size() > 3 ? it->second[3] : 0.0;
+    const bool ask = ask_tj || ask_seg || ask_chunk || ask_depth;
+    if (ask && it != cache.end()) {
+        const std::vector<double> &v = it->second;
+        const uint32_t depth = v.size() > 4 && uint32_t(v[4]) == 2 && b.pairs_ok ? 2 : 1;
+        if (ask_depth) b.depth = depth;
+        if (ask_tj) b.tj = uint32_t(v[0]) == 8 ? 8 : 14;
+        if (ask_seg) {
+            const uint32_t sg = std::min<uint32_t>(std::max<uint32_t>(uint32_t(v[1]), 1), SEG_MAX);
+            b.seg = b.seg_base = b.seg2 = b.seg2_base = sg;
+        }
+        if (ask_chunk) b.chunk = std::max<uint32_t>(uint32_t(v[2]), 1);
+        step_ms = v.size() > 3 ? v[3] : 0.0;
         from_cache = true;
-    } else if (ask_tj || ask_seg || ask_chunk) {
+    } else if (ask) {
         tuned = true;
-        // tile shape: direct launches, so only the kernels' own time differs. The first
-        // candidate sets how many steps each one gets: about 40 ms of stepping per candidate.
+        // tile shape and depth: direct launches, so only the kernels' own time differs. The
+        // first candidate sets how many steps each one gets: about 40 ms of stepping per
+        // candidate, an even number so two-step passes cover them exactly.
         double best = 1e30;
-        uint32_t btj = b.tj, bseg = b.seg, steps = 0;
-        for (uint32_t tj : {14u, 8u}) {
-            if (!ask_tj && tj != b.tj) continue;
-            for (uint32_t seg : {32u, 16u, 8u}) {
-                if (!ask_seg && seg != b.seg) continue;
-                b.tj = tj, b.seg = b.seg_base = seg;
-                plan(b, active);
-                if (!steps) steps = std::min<uint32_t>(8, std::max<uint32_t>(2, uint32_t(40.0 / time_steps(b, 1, 1, false))));
-                const double ms = time_steps(b, 1, steps, false);
-                if (ms < best) best = ms, btj = tj, bseg = seg;
+        uint32_t btj = b.tj, bseg = seg_in, bdepth = b.depth, steps = 0;
+        for (uint32_t depth : {1u, 2u}) {
+            if (!ask_depth && depth != b.depth) continue;
+            for (uint32_t tj : {14u, 8u}) {
+                if (depth == 2 && tj != 14) continue;  // the two-step march has one tile height
+                if (depth == 1 && !ask_tj && tj != b.tj) continue;
+                for (uint32_t seg : {32u, 16u, 8u}) {
+                    if (!ask_seg && seg != seg_in) continue;
+                    b.depth = depth, b.tj = tj, b.seg = b.seg_base = b.seg2 = b.seg2_base = seg;
+                    plan(b, active);
+                    if (!steps) {
+                        const double one = time_steps(b, 2, 2, false);
+                        steps = 2 * std::min<uint32_t>(4, std::max<uint32_t>(1, uint32_t(20.0 / one)));
+                    }
+                    const double ms = time_steps(b, 2, steps, false);
+                    if (ms < best) best = ms, btj = tj, bseg = seg, bdepth = depth;
+                }
             }
         }
-        b.tj = btj, b.seg = b.seg_base = bseg;
+        b.depth = bdepth, b.tj = btj, b.seg = b.seg_base = b.seg2 = b.seg2_base = bseg;
         plan(b, active);
         step_ms = best;
         if (ask_chunk) {
@@ -380,13 +507,20 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
         }
         FG_CHECK(hipStreamSynchronize(b.stream));
         for (auto &s : b.runs) reset(*s);
-        cache[key] = {double(b.tj), double(b.seg_base), double(b.chunk), step_ms};
+        b.par = 0;
+        cache[key] = {double(b.tj), double(b.depth == 2 ? b.seg2_base : b.seg_base), double(b.chunk), step_ms,
+                      double(b.depth)};
         write_cache(t ? t->cache : nullptr, cache);
     }
-    if (b.chunk > 1) b.chunk += b.chunk & 1u;  // replays start on even steps
+    // replays start and end on copy 0: an even number of passes per chunk
+    if (b.chunk > 1) {
+        const uint32_t m = b.depth == 2 ? 4 : 2;
+        b.chunk = (b.chunk + m - 1) / m * m;
+    }
     plan(b, active);
     if (t) {
-        t->used_tj = b.tj, t->used_seg = b.seg_base, t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0;
+        t->used_tj = b.depth == 2 ? TJ2 : b.tj, t->used_seg = b.depth == 2 ? b.seg2_base : b.seg_base;
+        t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0, t->used_depth = b.depth;
         t->step_ms = step_ms;
         t->tune_ms = tuned ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
     }
@@ -444,6 +578,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         b.cus = uint32_t(std::max(dev->cus, 1));
         std::vector<BatchRun> table(count);
         b.step.alloc(1);
+        bool all_pairs = true;
         for (uint32_t r = 0; r < count; r++) {
             if (!problems[r] || !extras[r]) throw HipError{"fg_run_batch: NULL problem or extras", FG_INVALID};
             fg_extras x = *extras[r];
@@ -456,7 +591,13 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             t.fix = s.fix;
             t.obs = s.obs;
             t.fix.step = t.obs.step = b.step.p;  // lockstep: one counter for the whole batch
+            t.sr = Src2{s.src2_start.p, s.src2_ent.p};
+            t.db = Deb2{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
+            t.fix2 = t.fix;
+            if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
+            if (!s.fuse2) all_pairs = false;
         }
+        b.pairs_ok = all_pairs && count > 0;
         b.table.upload(table.data(), table.size(), b.stream);
         std::vector<uint8_t> active(count, 0);
         for (uint32_t r = 0; r < count; r++) active[r] = extras[r]->max_steps > 0;
@@ -487,7 +628,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
                 if (!active[r]) continue;
                 Session &s = *b.runs[r];
                 s.steps = global;
-                s.cur = int(global & 1u);
+                s.cur = int(b.par);
                 if (global % every[r] && global != extras[r]->max_steps) continue;
                 const double e = energy(s);
                 if (!std::isfinite(e))  // run.rs:736-738

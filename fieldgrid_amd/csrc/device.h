@@ -462,7 +462,10 @@ struct FixArgs {
     const float *dtab;  // per set: np x (ka, kb)
     uint32_t np;
     float *dstate;      // np x n_debye, pole-major (run.rs:492-493)
-    const float *dold;  // split engine: E saved before the E sweep; fused: NULL (read Eo)
+    const float *dsrc;  // where the step reads the pole states when not from dstate (NULL =
+                        // in place): the second step of a two-step pass reads those the pass wrote
+    const float *dold;  // E before the step's E update: split engine, saved before the E sweep;
+                        // second step of a two-step pass, saved by the pass; else NULL (read Eo)
     uint32_t n_src;
     const fg_source *src;
     uint32_t n_ind;
@@ -495,15 +498,14 @@ __device__ __forceinline__ void fix_step(const Geo &g, const FixArgs &a, float *
         const float old = a.dold ? a.dold[t] : Eo[d.f];
         const float cb = coef_at(g, d.f).y;
         const float *tab = a.dtab + size_t(d.set) * a.np * 2;
+        const float *js = a.dsrc ? a.dsrc : a.dstate;
         float s0 = 0.f;
-        for (uint32_t q = 0; q < a.np; q++) s0 += 0.5f * (1.0f + tab[2 * q]) * a.dstate[q * a.n_debye + t];
+        for (uint32_t q = 0; q < a.np; q++) s0 += 0.5f * (1.0f + tab[2 * q]) * js[q * a.n_debye + t];
         const float next = E[d.f] - cb * s0;
         E[d.f] = next;
         const float de = next - old;
-        for (uint32_t q = 0; q < a.np; q++) {
-            float &st = a.dstate[q * a.n_debye + t];
-            st = tab[2 * q] * st + d.dd * tab[2 * q + 1] * de;
-        }
+        for (uint32_t q = 0; q < a.np; q++)
+            a.dstate[q * a.n_debye + t] = tab[2 * q] * js[q * a.n_debye + t] + d.dd * tab[2 * q + 1] * de;
     }
     auto source = [&](uint64_t q) {
         const uint32_t n = a.step[0] + s_off;
@@ -683,11 +685,24 @@ struct Src2 {
     const uint4 *ent;
 };
 
+// Step n's Debye ADE (cpu.rs:239-260) inside a two-step pass. Every copy of a Debye edge (owner
+// or halo) applies it from the pole states J^n, which nothing writes during the pass; the
+// edge's owner writes J^{n+1} and the edge's final step-n E to side buffers, from which the
+// sparse fixes of step n+1 read them. Edges are found per node through a k map of each (i, j)
+// row that holds any: row[i*n1 + j] is the row's slot or -1, map[(slot*n2 + k)*3 + c] the
+// Debye index of edge (c; i, j, k) or -1.
+struct Deb2 {
+    const int32_t *row;
+    const int32_t *map;
+    float *jn;  // J^{n+1}, pole-major like FixArgs::dstate
+    float *eo;  // step n's final E per Debye edge
+};
+
 template <int TJ, int PML, bool WIDE, bool LDS_SETS>
 __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
                                        float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
                                        const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
-                                       const Src2 &sr, float pl) {
+                                       const Src2 &sr, float pl, const FixArgs &fx, const Deb2 &db) {
     constexpr int ROWS = TJ + 4;
     constexpr bool PZ = PML >= 1, PXY = PML == 2;
     // row exchanges, double-buffered by plane parity; row w is stored at index w+1 so that
@@ -876,7 +891,30 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
             if constexpr (PXY) ezy = bey * ezy + cey * hu * rddy, t2 += ezy;
             nez = f2.y != 0.f ? f2.x * ez0 + f2.y * (t1 - t2) : ez0;
         }
-        // step n's port sources on this plane (every copy of the edge, owner or halo)
+        // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
+        // owner or halo), in the order of cpu.rs:284-297
+        const int drow = db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
+        int32_t dt3[3] = {-1, -1, -1};
+        if (drow >= 0) {
+            const int32_t *dm = db.map + (uint64_t(uint32_t(drow)) * g.n2 + uk) * 3;
+            for (int c = 0; c < 3; c++) dt3[c] = kin ? dm[c] : -1;
+            for (int c = 0; c < 3; c++) {
+                const int32_t di = dt3[c];
+                if (di < 0) continue;
+                const fg_debye_edge d = fx.debye[di];
+                const float cb = (c == 0 ? f0 : c == 1 ? f1 : f2).y;
+                const float *tab = fx.dtab + size_t(d.set) * fx.np * 2;
+                float s0 = 0.f;
+                for (uint32_t q = 0; q < fx.np; q++) s0 += 0.5f * (1.0f + tab[2 * q]) * fx.dstate[q * fx.n_debye + di];
+                float &e = c == 0 ? nex : c == 1 ? ney : nez;
+                const float next = e - cb * s0;
+                e = next;
+                const float de = next - (c == 0 ? ex0 : c == 1 ? ey0 : ez0);
+                if (own && t >= i0 && t < i1)
+                    for (uint32_t q = 0; q < fx.np; q++)
+                        db.jn[q * fx.n_debye + di] = tab[2 * q] * fx.dstate[q * fx.n_debye + di] + d.dd * tab[2 * q + 1] * de;
+            }
+        }
         if (t < g.n0) {
             const uint32_t s0 = sr.start[t], s1 = sr.start[t + 1];
             for (uint32_t q = s0; q < s1; q++) {
@@ -889,6 +927,9 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
                 }
             }
         }
+        if (drow >= 0 && own && t >= i0 && t < i1)
+            for (int c = 0; c < 3; c++)
+                if (dt3[c] >= 0) db.eo[dt3[c]] = c == 0 ? nex : c == 1 ? ney : nez;
         // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)
         const bool two = t >= 1 && t - 1 + 1 >= (i0 > 0 ? i0 : 1);
         const uint32_t s = t - 1;  // plane of step n+1
@@ -1073,6 +1114,43 @@ __device__ inline float field_mid(const Geo &g, const FixArgs &f, float pl, cons
         case 3: return h_mid<0>(g, E, H, psi, i, j, k);
         case 4: return h_mid<1>(g, E, H, psi, i, j, k);
         default: return h_mid<2>(g, E, H, psi, i, j, k);
+    }
+}
+
+// The first step of a two-step launch exists only inside it. Its observers (port V/I, point
+// probes) recompute the few values they read from the untouched previous copy (h_mid, e_mid,
+// field_mid above), block `blk` of the observer launch as in observe().
+__device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs &f, const float *__restrict__ E,
+                                   const float *__restrict__ H, const float *__restrict__ psi, uint32_t s_off,
+                                   uint32_t blk) {
+    const uint32_t n = a.step[0] + s_off;
+    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
+    const uint64_t nn = g.nn;
+    if (blk < a.blk_ports) {  // as observe(): one wave per port, f32 products summed in f64
+        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+        if (p >= a.n_ports) return;
+        const fg_port d = a.ports[p];
+        double v = 0.0, cur = 0.0;
+        for (uint32_t q = lane; q < d.n_edges; q += 64) {
+            const fg_port_edge pe = a.edges[d.first_edge + q];
+            v -= double(field_mid(g, f, pl, E, H, psi, d.comp * nn + pe.id) * pe.w);
+        }
+        for (uint32_t q = lane; q < d.n_loops; q += 64) {
+            const fg_loop_term lt = a.loops[d.first_loop + q];
+            cur += double(field_mid(g, f, pl, E, H, psi, (3 + lt.comp) * nn + lt.id) * lt.w);
+        }
+        v = wave_sum(v);
+        cur = wave_sum(cur);
+        if (lane == 0 && n < a.cap) {
+            a.series[(uint64_t(n) * a.n_ports + p) * 2] = float(v);
+            a.series[(uint64_t(n) * a.n_ports + p) * 2 + 1] = float(cur);
+        }
+        return;
+    }
+    blk -= a.blk_ports;
+    if (blk < a.blk_probes) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = field_mid(g, f, pl, E, H, psi, a.probe_f[q]);
     }
 }
 

@@ -102,50 +102,20 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
                                                           const float *__restrict__ Eo, float *__restrict__ En,
                                                           const float *__restrict__ Ho, float *__restrict__ Hn,
                                                           const float *__restrict__ psio, float *__restrict__ psin,
-                                                          uint32_t seg, Src2 sr, FixArgs f, uint32_t s_off) {
+                                                          uint32_t seg, Src2 sr, FixArgs f, Deb2 db, uint32_t s_off) {
     const MarchBlock mb = blocks[blockIdx.x];
     if (mb.bx == ~0u) return;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    march2<TJ, PML, WIDE, LDS_SETS>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl);
+    march2<TJ, PML, WIDE, LDS_SETS>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db);
 }
 
-// The first step of a two-step launch exists only inside it. Its observers (port V/I, point
-// probes) recompute the few values they read from the untouched previous copy (h_mid, e_mid,
-// field_mid in device.h).
+// The first step of a two-step launch exists only inside it; its observers recompute what they
+// read from the untouched previous copy (observe_mid).
 __global__ __launch_bounds__(256) void k_observe_mid(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ E,
                                                      const float *__restrict__ H, const float *__restrict__ psi,
                                                      uint32_t s_off) {
-    const uint32_t n = a.step[0] + s_off;
-    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    const uint64_t nn = g.nn;
-    uint32_t blk = blockIdx.x;
-    if (blk < a.blk_ports) {  // as observe(): one wave per port, f32 products summed in f64
-        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
-        if (p >= a.n_ports) return;
-        const fg_port d = a.ports[p];
-        double v = 0.0, cur = 0.0;
-        for (uint32_t q = lane; q < d.n_edges; q += 64) {
-            const fg_port_edge pe = a.edges[d.first_edge + q];
-            v -= double(field_mid(g, f, pl, E, H, psi, d.comp * nn + pe.id) * pe.w);
-        }
-        for (uint32_t q = lane; q < d.n_loops; q += 64) {
-            const fg_loop_term lt = a.loops[d.first_loop + q];
-            cur += double(field_mid(g, f, pl, E, H, psi, (3 + lt.comp) * nn + lt.id) * lt.w);
-        }
-        v = wave_sum(v);
-        cur = wave_sum(cur);
-        if (lane == 0 && n < a.cap) {
-            a.series[(uint64_t(n) * a.n_ports + p) * 2] = float(v);
-            a.series[(uint64_t(n) * a.n_ports + p) * 2 + 1] = float(cur);
-        }
-        return;
-    }
-    blk -= a.blk_ports;
-    if (blk < a.blk_probes) {
-        const uint32_t q = blk * 256 + threadIdx.x;
-        if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = field_mid(g, f, pl, E, H, psi, a.probe_f[q]);
-    }
+    observe_mid(g, a, f, E, H, psi, s_off, blockIdx.x);
 }
 
 // ---- energy: sum E^2 + (376.73 H)^2 (cpu.rs:417-421, fdtd.wgsl:547-568) -----------------
@@ -522,9 +492,26 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     s.nmax = nmax;
     s.slots_host = axs;
     if (s.fused) plan_march(s);
-    // two steps per launch when nothing but port sources acts between steps
+    // two steps per launch when only port sources and Debye media act between steps, and no
+    // observer of the first step reads a Debye edge (the first step's observers recompute their
+    // values from the previous copy without the ADE)
+    bool observed_debye = false;
+    if (P->n_debye) {
+        std::vector<uint64_t> dfl(P->n_debye);
+        for (uint64_t t = 0; t < P->n_debye; t++) dfl[t] = P->debye_edges[t].f;
+        std::sort(dfl.begin(), dfl.end());
+        auto is_debye = [&](uint64_t flat) { return std::binary_search(dfl.begin(), dfl.end(), flat); };
+        for (uint32_t p = 0; p < P->n_ports && !observed_debye; p++)
+            for (uint32_t q = 0; q < P->ports[p].n_edges; q++)
+                if (is_debye(P->ports[p].comp * nn + P->port_edges[P->ports[p].first_edge + q].id)) observed_debye = true;
+        for (uint32_t q = 0; q < X->n_probes; q++) {
+            const fg_point_probe &pp = X->probes[q];
+            if (pp.field < 3 && is_debye(pp.field * nn + (uint64_t(pp.i) * P->n[1] + pp.j) * P->n[2] + pp.k))
+                observed_debye = true;
+        }
+    }
     const char *f2 = getenv("FIELDGRID_FUSE2");
-    s.fuse2 = s.fused && !(f2 && f2[0] == '0') && f.n_debye == 0 && f.n_ind == 0 && !f.serial && o.plane_n == 0 &&
+    s.fuse2 = s.fused && !(f2 && f2[0] == '0') && !observed_debye && f.n_ind == 0 && !f.serial && o.plane_n == 0 &&
               o.n_patches == 0;
     if (s.fuse2) {
         s.seg2 = 32;
@@ -556,6 +543,24 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         s.src2_start.upload(start.data(), start.size(), st);
         if (ent.empty()) s.src2_ent.alloc(1);
         else s.src2_ent.upload(ent.data(), ent.size(), st);
+        // the Debye edges' k maps, one per (i, j) row that holds any
+        if (P->n_debye) {
+            if (P->n_debye >= (1ull << 31)) throw HipError{"too many Debye edges for the two-step march", FG_TOO_LARGE};
+            std::vector<int32_t> row(uint64_t(P->n[0]) * P->n[1], -1), map;
+            for (uint64_t t = 0; t < P->n_debye; t++) {
+                const uint64_t fl = P->debye_edges[t].f, c = fl / nn, id = fl % nn;
+                const uint64_t r = id / P->n[2], k = id % P->n[2];
+                if (row[r] < 0) {
+                    row[r] = int32_t(map.size() / (3ull * P->n[2]));
+                    map.resize(map.size() + 3ull * P->n[2], -1);
+                }
+                map[(uint64_t(row[r]) * P->n[2] + k) * 3 + c] = int32_t(t);
+            }
+            s.deb2_row.upload(row.data(), row.size(), st);
+            s.deb2_map.upload(map.data(), map.size(), st);
+            s.deb2_jn.alloc(std::max<uint64_t>(uint64_t(P->n_poles) * P->n_debye, 1));
+            s.deb2_eo.alloc(P->n_debye);
+        }
         plan_march2(s);
     }
     FG_CHECK(hipStreamSynchronize(st));
@@ -684,11 +689,12 @@ void launch_pair(Session &s, uint32_t s_off) {
     const uint32_t ni = uint32_t(s.blk2_int.n), nz = uint32_t(s.blk2_z.n), np_ = uint32_t(s.blk2_pml.n);
     const MarchBlock *bi = s.blk2_int.p, *bz = s.blk2_z.p, *bp = s.blk2_pml.p;
     const Src2 sr{s.src2_start.p, s.src2_ent.p};
+    const Deb2 db{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
 #define FG_MARCH2X(TJ, W, L)                                                                                         \
     do {                                                                                                             \
-        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
-        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
-        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, s_off); \
+        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
     } while (0)
     {
         const bool lds = g.n_sets <= uint32_t(SETS_LDS);
@@ -701,7 +707,10 @@ void launch_pair(Session &s, uint32_t s_off) {
 #undef FG_MARCH2X
     s.cur = b;
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
-    if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, s.fix, En, Eo, s_off + 1);
+    // the second step's sparse fixes: Debye from the pole states and step-n E the pass wrote
+    FixArgs f2 = s.fix;
+    if (s.fix.n_debye) f2.dsrc = db.jn, f2.dold = db.eo;
+    if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, f2, En, Eo, s_off + 1);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 2), st));
     if (s.obs_blocks && s.observe) {
         hipLaunchKernelGGL(k_observe_mid, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, s_off);

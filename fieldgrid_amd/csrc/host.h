@@ -109,6 +109,8 @@ struct Session {
     DevBuf<MarchBlock> blk2_int, blk2_z, blk2_pml;
     DevBuf<uint32_t> src2_start;
     DevBuf<uint4> src2_ent;
+    DevBuf<int32_t> deb2_row, deb2_map;  // Debye k maps of the two-step march (Deb2)
+    DevBuf<float> deb2_jn, deb2_eo;
     std::vector<std::pair<size_t, uint32_t>> units;  // timed launch units: (event slot, steps)
     std::vector<hipEvent_t> events;
     // host copies the batched sweep and reset() need

@@ -557,12 +557,14 @@ class BatchRun:
 
 @dataclass
 class Tune:
-    """March tile and graph chunk for a batch (0 = autotune), with an optional JSON cache."""
+    """March tile, steps per march pass (1 or 2) and graph chunk for a batch (0 = autotune),
+    with an optional JSON cache. `used` comes back as (tj, seg, chunk, depth)."""
     tj: int = 0
     seg: int = 0
     chunk: int = 0
+    depth: int = 0
     cache: Optional[str] = None
-    used: Tuple[int, int, int] = (0, 0, 0)
+    used: Tuple[int, int, int, int] = (0, 0, 0, 0)
     from_cache: bool = False
     tune_ms: float = 0.0
     step_ms: float = 0.0
@@ -609,7 +611,7 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
             spec_bufs.append(None)
     t = abi.fg_tune()
     if tune is not None:
-        t.tj, t.seg, t.chunk = tune.tj, tune.seg, tune.chunk
+        t.tj, t.seg, t.chunk, t.depth = tune.tj, tune.seg, tune.chunk, tune.depth
         t.cache = tune.cache.encode() if tune.cache else None
     errors = []
 
@@ -626,7 +628,7 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
         raise errors[0]
     abi.check(status)
     if tune is not None:
-        tune.used = (t.used_tj, t.used_seg, t.used_chunk)
+        tune.used = (t.used_tj, t.used_seg, t.used_chunk, t.used_depth)
         tune.from_cache = bool(t.from_cache)
         tune.tune_ms, tune.step_ms = t.tune_ms, t.step_ms
     records = [_finish(recs[r], bufs_all[r], runs[r].sim, runs[r].extras) for r in range(count)]

@@ -8,11 +8,12 @@
  * outputs are those a single fg_run of it gives. It adds port V/I spectra computed on the
  * device, as the sims-antenna caller does on the host (crates/sims-antenna/src/fdtd/mod.rs:229-266).
  *
- * All runs advance together from step 0. Every step is one launch per block class of the fused
+ * All runs advance together from step 0. Every pass is one launch per block class of the fused
  * march over the blocks of every run still going, plus one launch of the sparse fixes and one
- * of the observers for all of them. Chunks of steps are captured in a hipGraph and replayed.
- * The march's tile shape (rows per block, planes per block) and the graph chunk length come
- * from an autotuner, whose choices can be cached in a JSON file.
+ * of the observers for all of them; a pass advances one step, or two (temporal blocking) where
+ * every run allows it. Chunks of passes are captured in a hipGraph and replayed. The march's
+ * tile shape (rows per block, planes per block), the steps per pass and the graph chunk length
+ * come from an autotuner, whose choices can be cached in a JSON file.
  */
 #ifndef FIELDGRID_BATCH_H
 #define FIELDGRID_BATCH_H
@@ -27,13 +28,16 @@ extern "C" {
 typedef struct fg_tune {
     uint32_t tj;          /* rows per march block, 8 or 14; 0 = tune                           */
     uint32_t seg;         /* i planes per march block, 1..64; 0 = tune                         */
-    uint32_t chunk;       /* steps per captured hipGraph (rounded up to even); 0 = tune;
-                             1 = no graph (every step launched directly)                       */
-    uint32_t pad;
+    uint32_t chunk;       /* steps per captured hipGraph (rounded up to an even number of
+                             passes); 0 = tune; 1 = no graph (every pass launched directly)    */
+    uint32_t depth;       /* steps per march pass (step-fusion depth): 1, or 2 = two steps per
+                             pass over HBM where every run allows it (no inductors, plane DFT,
+                             NTFF, and no observed Debye edge); 0 = tune                        */
     const char *cache;    /* JSON file of tuned choices per batch shape: read if present and
                              updated after tuning; NULL = no cache                              */
     /* out */
     uint32_t used_tj, used_seg, used_chunk, from_cache;
+    uint32_t used_depth, pad;
     double tune_ms;       /* time spent tuning (0 when the cache or the caller decided)         */
     double step_ms;       /* mean time of one step of the whole batch, as the tuner measured it */
 } fg_tune;

@@ -89,15 +89,22 @@ def _same(a, b, what):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tune", [F.Tune(tj=14, seg=16, chunk=8), F.Tune(tj=8, seg=8, chunk=1), F.Tune()],
-                         ids=["tj14-seg16-graph8", "tj8-seg8-direct", "autotuned"])
+@pytest.mark.parametrize("tune", [F.Tune(tj=14, seg=16, chunk=8, depth=1), F.Tune(tj=8, seg=8, chunk=1, depth=1),
+                                  F.Tune(seg=16, chunk=8, depth=2), F.Tune(seg=3, chunk=1, depth=2), F.Tune()],
+                         ids=["tj14-seg16-graph8", "tj8-seg8-direct", "depth2-seg16-graph8", "depth2-seg3-direct",
+                              "autotuned"])
 def test_batched_runs_equal_single_runs(gpu, tune):
+    """Every batched record bit-identical to a single fg_run of the same problem, with one step
+    per march pass and with two (the runs have Debye boxes: the first step's ADE runs inside the
+    pass). Odd step counts and check_every 40 end chunks on single steps and shrink the lists."""
     runs = small_sweep()
     for r in runs:
         r.extras.check_every = 40  # several decay checks and list rebuilds per run
     print(f"\nbatch of {len(runs)}: dims {[r.sim.dims() for r in runs]}, steps {[r.extras.max_steps for r in runs]}")
     recs, spectra = F.run_batch(gpu, runs, tune=tune, fields=True)
-    print(f"  tuned: tj, seg, chunk = {tune.used}, tune {tune.tune_ms:.1f} ms, step {tune.step_ms:.3f} ms")
+    print(f"  tuned: tj, seg, chunk, depth = {tune.used}, tune {tune.tune_ms:.1f} ms, step {tune.step_ms:.3f} ms")
+    if tune.depth:
+        assert tune.used[3] == tune.depth
     for q, r in enumerate(runs):
         single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
         _same(recs[q], single, f"run {q}")

@@ -220,6 +220,42 @@ def test_debye_boxes_match_the_oracle(gpu, per_box_tau):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("per_box_tau", [False, True], ids=["global-pole", "pole-per-box"])
+@pytest.mark.parametrize("seg2", ["32", "3"])
+def test_two_steps_per_launch_with_debye_media_change_nothing(gpu, monkeypatch, seg2, per_box_tau):
+    """march2 with the first step's Debye ADE inside the pass (every copy of a Debye edge applies
+    it from the pole states; the owner writes the new states and E for the second step's sparse
+    fixes): the boxes recipe at 56^3 without NTFF, against one step per pass and the FP32 oracle,
+    bit for bit. An E probe on a Debye edge makes the run fall back to one step per pass (the
+    first step's observers recompute without the ADE), which must change nothing either."""
+    sim = W.boxes(nodes=56, pml=6, nboxes=8, ndebye=4, per_box_tau=per_box_tau)
+    p = sim.problem()
+    assert p.n_debye > 0
+    pulse = W.boxes_pulse()
+    n = sim.dims()
+    nn = n[0] * n[1] * n[2]
+    debye = {int(p.debye_edges[t].f) for t in range(p.n_debye)}
+    c, rest = divmod(min(debye), nn)
+    on_debye = (c, rest // (n[1] * n[2]), (rest // n[2]) % n[1], rest % n[2])
+    # the first probe set and the port stay off Debye edges, so that run takes two steps per pass
+    port_edges = {int(p.ports[q].comp) * nn + int(p.port_edges[p.ports[q].first_edge + e].id)
+                  for q in range(p.n_ports) for e in range(p.ports[q].n_edges)}
+    assert not port_edges & debye
+    assert not {2 * nn + (30 * n[1] + 28) * n[2] + 20} & debye
+    monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    for probes in ([(2, 30, 28, 20), (3, 25, 30, 31)], [(2, 30, 28, 20), on_debye]):
+        ex = F.Extras(max_steps=351, decay_db=math.inf, check_every=100, probes=probes)
+        monkeypatch.delenv("FIELDGRID_FUSE2", raising=False)
+        two = F.run(gpu, sim, 0, pulse, ex, fields=True)
+        monkeypatch.setenv("FIELDGRID_FUSE2", "0")
+        one = F.run(gpu, sim, 0, pulse, ex, fields=True)
+        for name in ("series", "probes", "e", "h"):
+            a, b = getattr(two, name), getattr(one, name)
+            assert np.array_equal(a, b), f"{name}: {int(np.sum(a != b))} of {a.size} differ"
+        _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"boxes seg2 {seg2} probes {probes}")
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("seg2", ["32", "5", "1"])
 def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2):
     """march2 (two steps per pass, the intermediate step in registers) against one step per
