@@ -74,7 +74,7 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun 
     const BatchRun &r = runs[bb.run];
     const uint32_t n = r.fix.step[0] + s_off;
     const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
-    march2<TJ2, PML, WIDE, LDS_SETS>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
+    march2<TJ2, PML, WIDE, LDS_SETS, true>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
                                      r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db);
 }
 

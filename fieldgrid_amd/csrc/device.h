@@ -674,8 +674,8 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
 // neighbours computed (the same loads and the same arithmetic). Along i the block marches
 // with a two-plane lag: iteration t computes step n at plane t and step n+1 at plane t-1.
 // Step n's port sources are applied in the kernel, to every copy of the edge (owner or halo),
-// from a per-plane list; nothing else may sit between the two steps (no Debye media,
-// inductors, plane DFT or NTFF: those runs take one step per pass).
+// from a per-plane list, and so is its Debye ADE (DEB: Deb2 below); nothing else may sit between
+// the two steps (no inductors, plane DFT or NTFF: those runs take one step per pass).
 constexpr int OWN_K2 = 60;
 
 // Step n's sources by plane: entries of plane i are ent[start[i] .. start[i+1]) as
@@ -698,7 +698,7 @@ struct Deb2 {
     float *eo;  // step n's final E per Debye edge
 };
 
-template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool DEB>
 __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
                                        float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
                                        const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
@@ -893,9 +893,9 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
         }
         // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
         // owner or halo), in the order of cpu.rs:284-297
-        const int drow = db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
+        const int drow = DEB && db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
         int32_t dt3[3] = {-1, -1, -1};
-        if (drow >= 0) {
+        if (DEB && drow >= 0) {
             const int32_t *dm = db.map + (uint64_t(uint32_t(drow)) * g.n2 + uk) * 3;
             for (int c = 0; c < 3; c++) dt3[c] = kin ? dm[c] : -1;
             for (int c = 0; c < 3; c++) {
@@ -927,7 +927,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
                 }
             }
         }
-        if (drow >= 0 && own && t >= i0 && t < i1)
+        if (DEB && drow >= 0 && own && t >= i0 && t < i1)
             for (int c = 0; c < 3; c++)
                 if (dt3[c] >= 0) db.eo[dt3[c]] = c == 0 ? nex : c == 1 ? ney : nez;
         // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_observe(Geo g, ObsArgs a, const float *
 
 // Two steps of the march per launch (see march2): the first step's pulse value comes from the
 // device step counter and the host table, as in k_fix.
-template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool DEB>
 __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBlock *__restrict__ blocks,
                                                           const float *__restrict__ Eo, float *__restrict__ En,
                                                           const float *__restrict__ Ho, float *__restrict__ Hn,
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
     if (mb.bx == ~0u) return;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    march2<TJ, PML, WIDE, LDS_SETS>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db);
+    march2<TJ, PML, WIDE, LDS_SETS, DEB>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db);
 }
 
 // The first step of a two-step launch exists only inside it; its observers recompute what they
@@ -690,20 +690,27 @@ void launch_pair(Session &s, uint32_t s_off) {
     const MarchBlock *bi = s.blk2_int.p, *bz = s.blk2_z.p, *bp = s.blk2_pml.p;
     const Src2 sr{s.src2_start.p, s.src2_ent.p};
     const Deb2 db{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
-#define FG_MARCH2X(TJ, W, L)                                                                                         \
+#define FG_MARCH2X(TJ, W, L, D)                                                                                      \
     do {                                                                                                             \
-        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
-        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
-        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L, D>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L, D>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+    } while (0)
+#define FG_MARCH2D(W, L)                                     \
+    do {                                                    \
+        if (db.row) FG_MARCH2X(12, W, L, true);             \
+        else FG_MARCH2X(12, W, L, false);                   \
     } while (0)
     {
+        // runs without Debye media take the kernel without the in-pass ADE (fewer registers)
         const bool lds = g.n_sets <= uint32_t(SETS_LDS);
         if (g.wide) {
-            if (lds) FG_MARCH2X(12, true, true); else FG_MARCH2X(12, true, false);
+            if (lds) FG_MARCH2D(true, true); else FG_MARCH2D(true, false);
         } else {
-            if (lds) FG_MARCH2X(12, false, true); else FG_MARCH2X(12, false, false);
+            if (lds) FG_MARCH2D(false, true); else FG_MARCH2D(false, false);
         }
     }
+#undef FG_MARCH2D
 #undef FG_MARCH2X
     s.cur = b;
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
