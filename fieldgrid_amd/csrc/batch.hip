@@ -581,6 +581,8 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         bool all_pairs = true;
         for (uint32_t r = 0; r < count; r++) {
             if (!problems[r] || !extras[r]) throw HipError{"fg_run_batch: NULL problem or extras", FG_INVALID};
+            if (problems[r]->n_sheets || problems[r]->n_wires)
+                throw HipError{"fg_run_batch: resistive sheets and thin wires run through fg_run", FG_UNSUPPORTED};
             fg_extras x = *extras[r];
             x.engine = FG_ENGINE_FUSED;
             b.runs.emplace_back(session_new(dev, problems[r], driven[r], &pulses[r], &x));

@@ -190,11 +190,23 @@ __device__ __forceinline__ float cpml(const float *po, float *pn, uint32_t bytes
 }
 
 
-// PML: 0 = no CPML slab in reach, 1 = only z (k) slabs, 2 = any slab (x, y and z psi code)
-template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+// Thin wires (cpu.rs:270-273, sim.rs:556-635): after the H update, H[h] += sum_t c_t E[e_t], E
+// being the step's input. In the fused march every copy of such an H node (owner or halo) adds
+// it, so the E updates of every block see the same H. Entries of plane i are
+// ent[start[i] .. start[i+1]) as (j, k, component, index into w).
+struct WireArgs {
+    const uint32_t *start;
+    const uint4 *ent;
+    const fg_wire_h *w;
+};
+
+// PML: 0 = no CPML slab in reach, 1 = only z (k) slabs, 2 = any slab (x, y and z psi code);
+// WIRES: the step has thin-wire terms (wa)
+template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool WIRES = false>
 __device__ __forceinline__ void march(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
                                       float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
-                                      const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg) {
+                                      const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
+                                      const WireArgs &wa = WireArgs{}) {
     constexpr int ROWS = TJ + 2;
     // row exchanges; rows are stored at index w+1 so that rows -1 and ROWS are zero pads
     __shared__ float xe[2][ROWS + 2][2][64];  // E(i+1) Ex, Ez of each row, for the row above
@@ -387,6 +399,20 @@ __device__ __forceinline__ void march(const Geo &g, const MarchBlock mb, const f
             if constexpr (PXY) t2 += CP(PSI(1, 2, 1) + yrow, ybytes, v ? oyh : OOB, bhy, chy, rdy, hzy, eu);
             hz = il && jl ? cur.hz - g.k_mu * (t1 - t2) : cur.hz;
         }
+        if constexpr (WIRES) {
+            const uint32_t q0 = wa.start[i], q1 = wa.start[i + 1];  // uniform
+            for (uint32_t q = q0; q < q1; q++) {
+                const uint4 e = wa.ent[q];
+                if (act && e.x == uj && e.y == uk) {
+                    const fg_wire_h &wh = wa.w[e.w];
+                    float sw = 0.f;
+                    for (int t = 0; t < 4; t++) sw += wh.c[t] * Eo[wh.e[t]];
+                    if (e.z == 0) hx += sw;
+                    else if (e.z == 1) hy += sw;
+                    else hz += sw;
+                }
+            }
+        }
         {
             float *q = Hn + pb;
             st(q, sto, pl4, hx), st(q + nn, sto, pl4, hy), st(q + 2 * nn, sto, pl4, hz);
@@ -529,6 +555,79 @@ __device__ __forceinline__ void fix_step(const Geo &g, const FixArgs &a, float *
     }
     if (t < a.n_src) source(t);
     if (t < a.n_ind) inductor(t);
+}
+
+// ---- resistive sheets (cpu.rs:169-237, fdtd.wgsl:332-427) -------------------------------
+
+// One sheet edge's state lives in its 16-f32 record (the pack() layout, run.rs:259-290): [0] E
+// index, [1], [2] the H indices on either face, [3] length, [4] c, [5] r, [6] adaptive, [7..9]
+// running aa, bb, ab, [10], [11] face weights, [12] E before the E update, [13] the previous
+// current, [14], [15] the faces' rows in the sheet table. The table holds nr real and nc
+// complex branches: per branch its recursion coefficients, per face row its weights.
+struct SheetArgs {
+    uint64_t n;
+    float *sh;
+    float *cur;         // n x (nr + 2 nc) branch currents
+    const float *br;
+    uint32_t serial;    // sheets share an edge: one thread applies them in order
+};
+
+// cpu.rs:169-193: face weights from the running H energy on either face
+__device__ __forceinline__ void sheet_before(const SheetArgs &a, uint64_t s, float old, float ha, float hb) {
+    float *sh = a.sh + s * 16;
+    sh[12] = old;
+    const float aa = sh[7] + ha * ha, bb = sh[8] + hb * hb, ab = sh[9] + ha * hb;
+    sh[7] = aa, sh[8] = bb, sh[9] = ab;
+    const float sum = aa + bb;
+    if (sum <= 1e-30f) return;
+    float scale = 1.0f;
+    if (sh[6] != 0.0f) {
+        scale = sh[10] + sh[11];
+        const float tot = aa + bb - 2.0f * ab;
+        if (tot > 1e-30f) {
+            float x = sum / tot;  // clamp(x, 0.5, 4.0), NaN passing through as in Rust
+            x = x < 0.5f ? 0.5f : x;
+            scale = 4.0f < x ? 4.0f : x;
+        }
+    }
+    sh[10] = scale * aa / sum;
+    sh[11] = scale * bb / sum;
+}
+
+// cpu.rs:195-237: the sheet's current from the trapezoidal branch model, taken out of E
+__device__ __forceinline__ void sheet_after(const SheetArgs &a, uint64_t s, float *__restrict__ E) {
+    float *sh = a.sh + s * 16;
+    const float *br = a.br;
+    const uint64_t f = __float_as_uint(sh[0]);
+    const float len = sh[3], c = sh[4], r = sh[5];
+    const float ma = sh[10], mb = sh[11], old = sh[12], i_old = sh[13];
+    const uint32_t nr = uint32_t(br[0]), nc = uint32_t(br[1]);
+    const uint32_t stride = 2 + nr + 2 * nc, faces = 4 + 2 * nr + 4 * nc;
+    const uint64_t fa = faces + uint64_t(__float_as_uint(sh[14])) * stride,
+                   fb = faces + uint64_t(__float_as_uint(sh[15])) * stride;
+    float *cu = a.cur + s * (nr + 2 * nc);
+    float hist = (ma * br[fa + 1] + mb * br[fb + 1]) * i_old;
+    for (uint32_t k = 0; k < nr; k++) hist -= (ma * br[fa + 2 + k] + mb * br[fb + 2 + k]) * cu[k];
+    for (uint32_t k = 0; k < nc; k++) {
+        const uint32_t o = 2 + nr + 2 * k;
+        const float cr = ma * br[fa + o] + mb * br[fb + o];
+        const float ci = ma * br[fa + o + 1] + mb * br[fb + o + 1];
+        hist -= cr * cu[nr + 2 * k] - ci * cu[nr + 2 * k + 1];
+    }
+    const float ra = r * (ma * br[fa] + mb * br[fb]);
+    const float ibar = (0.5f * len * (E[f] + old) + r * hist) / (ra + 0.5f * len * c);
+    E[f] = E[f] - c * ibar;
+    sh[13] = 2.0f * ibar - i_old;
+    for (uint32_t k = 0; k < nr; k++) {
+        const uint32_t q = 4 + 2 * k;
+        cu[k] = br[q] * cu[k] + br[q + 1] * ibar;
+    }
+    for (uint32_t k = 0; k < nc; k++) {
+        const uint32_t q = 4 + 2 * nr + 4 * k;
+        const float ur = cu[nr + 2 * k], ui = cu[nr + 2 * k + 1];
+        cu[nr + 2 * k] = br[q] * ur - br[q + 1] * ui + br[q + 2] * ibar;
+        cu[nr + 2 * k + 1] = br[q] * ui + br[q + 1] * ur + br[q + 3] * ibar;
+    }
 }
 
 // ---- per-step observers: port V/I, point probes, plane DFT, NTFF (cpu.rs:309-415) --------

@@ -79,6 +79,74 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_march(Geo g, const MarchBlock
     march<TJ, PML, WIDE, LDS_SETS>(g, mb, Eo, En, Ho, Hn, psio, psin, seg);
 }
 
+// The same sweep for a problem with thin wires (always 14 rows per block)
+template <int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * 16) void k_march_wires(Geo g, const MarchBlock *__restrict__ blocks,
+                                                         const float *__restrict__ Eo, float *__restrict__ En,
+                                                         const float *__restrict__ Ho, float *__restrict__ Hn,
+                                                         const float *__restrict__ psio, float *__restrict__ psin,
+                                                         uint32_t seg, WireArgs wa) {
+    const MarchBlock mb = blocks[blockIdx.x];
+    if (mb.bx == ~0u) return;
+    march<14, PML, WIDE, LDS_SETS, true>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, wa);
+}
+
+// Thin wires, split engine: H[h] += sum_t c_t E[e_t] after the H sweep; one thread per entry, or
+// one thread for all in order where entries share an H node (`serial`)
+__global__ __launch_bounds__(256) void k_wires(const fg_wire_h *__restrict__ w, uint64_t n, uint32_t serial,
+                                               const float *__restrict__ E, float *__restrict__ H) {
+    const uint64_t t = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    auto one = [&](uint64_t q) {
+        const fg_wire_h &wh = w[q];
+        float sw = 0.f;
+        for (int u = 0; u < 4; u++) sw += wh.c[u] * E[wh.e[u]];
+        H[wh.h] += sw;
+    };
+    if (serial) {
+        if (t == 0)
+            for (uint64_t q = 0; q < n; q++) one(q);
+    } else if (t < n) {
+        one(t);
+    }
+}
+
+// Resistive sheets after a fused sweep (fdtd.wgsl's sheet_fix): the face weights from the step's
+// input E and new H, then the sheet current out of the new E.
+__global__ __launch_bounds__(256) void k_sheet(SheetArgs a, float *__restrict__ En, const float *__restrict__ Eo,
+                                               const float *__restrict__ Hn) {
+    const uint64_t t = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    auto one = [&](uint64_t s) {
+        const float *sh = a.sh + s * 16;
+        sheet_before(a, s, Eo[__float_as_uint(sh[0])], Hn[__float_as_uint(sh[1])], Hn[__float_as_uint(sh[2])]);
+        sheet_after(a, s, En);
+    };
+    if (a.serial) {
+        if (t == 0)
+            for (uint64_t s = 0; s < a.n; s++) one(s);
+    } else if (t < a.n) {
+        one(t);
+    }
+}
+
+// Split engine: the face weights between the H and E sweeps (E still the step's input) ...
+__global__ __launch_bounds__(256) void k_sheet_pre(SheetArgs a, const float *__restrict__ E, const float *__restrict__ H) {
+    const uint64_t s = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (s >= a.n) return;
+    const float *sh = a.sh + s * 16;
+    sheet_before(a, s, E[__float_as_uint(sh[0])], H[__float_as_uint(sh[1])], H[__float_as_uint(sh[2])]);
+}
+
+// ... and the sheet currents after the E sweep
+__global__ __launch_bounds__(256) void k_sheet_post(SheetArgs a, float *__restrict__ E) {
+    const uint64_t t = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (a.serial) {
+        if (t == 0)
+            for (uint64_t s = 0; s < a.n; s++) sheet_after(a, s, E);
+    } else if (t < a.n) {
+        sheet_after(a, t, E);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_debye_save(uint64_t n, const fg_debye_edge *__restrict__ d,
                                                      const float *__restrict__ E, float *__restrict__ old) {
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -180,6 +248,31 @@ static void validate(const fg_problem *P, uint32_t driven, const fg_extras *X) {
     for (uint64_t q = 0; q < P->n_debye; q++)
         if (P->debye_edges[q].f >= 3 * nn || P->debye_edges[q].set >= std::max(1u, P->n_pole_sets))
             bad("a Debye edge lies outside the grid");
+    if (P->n_wires && !P->wires) bad("thin-wire table is NULL");
+    for (uint64_t q = 0; q < P->n_wires; q++) {
+        const fg_wire_h &w = P->wires[q];
+        if (w.h >= 3 * nn || w.e[0] >= 3 * nn || w.e[1] >= 3 * nn || w.e[2] >= 3 * nn || w.e[3] >= 3 * nn)
+            bad("a thin-wire term lies outside the grid");
+    }
+    if (P->n_sheets) {
+        // the sheet table: [nr, nc, ..] then per face a row of 2 + nr + 2 nc weights (run.rs:259-290)
+        if (!P->sheets || !P->branches || P->n_branches < 4) bad("resistive sheet tables are NULL or short");
+        const float fnr = P->branches[0], fnc = P->branches[1];
+        if (!(fnr >= 0.f && fnc >= 0.f && fnr < 65536.f && fnc < 65536.f) || fnr != float(uint32_t(fnr)) ||
+            fnc != float(uint32_t(fnc)))
+            bad("the sheet table's branch counts are not small whole numbers");
+        const uint64_t nr = uint32_t(fnr), nc = uint32_t(fnc), stride = 2 + nr + 2 * nc, faces = 4 + 2 * nr + 4 * nc;
+        if (P->sheet_states < nr + 2 * nc) bad("sheet_states is smaller than the sheet table's branches");
+        for (uint64_t q = 0; q < P->n_sheets; q++) {
+            const float *sh = P->sheets + 16 * q;
+            uint32_t id[3], fc[2];
+            std::memcpy(id, sh, 12);
+            std::memcpy(fc, sh + 14, 8);
+            if (id[0] >= 3 * nn || id[1] >= 3 * nn || id[2] >= 3 * nn) bad("a resistive sheet edge lies outside the grid");
+            for (uint32_t f : fc)
+                if (faces + (uint64_t(f) + 1) * stride > P->n_branches) bad("a resistive sheet's face row is past the sheet table");
+        }
+    }
 }
 
 static uint64_t psi_len(const uint32_t n[3], int a, uint32_t pml) {
@@ -269,8 +362,6 @@ static void plan_march(Session &s) {
 Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse,
                             const fg_extras *X) {
     validate(P, driven, X);
-    if (P->n_sheets || P->n_wires)
-        throw HipError{"resistive sheets and thin wires run on the CPU path (run_cpu)", FG_UNSUPPORTED};
     auto S = std::make_unique<Session>();
     Session &s = *S;
     s.dev = dev;
@@ -368,6 +459,44 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         s.ind_i0 = i0;
     }
     f.ind = s.ind.p;
+    // resistive sheets: records (state updated in place, restored by reset), branch currents
+    if (P->n_sheets) {
+        s.sheets0.assign(P->sheets, P->sheets + 16 * P->n_sheets);
+        s.sheets.upload(P->sheets, 16 * P->n_sheets, st);
+        s.branches.upload(P->branches, P->n_branches, st);
+        const uint64_t nb = uint64_t(P->branches[0]) + 2 * uint64_t(P->branches[1]);
+        s.sheet_cur.alloc(std::max<uint64_t>(P->n_sheets * nb, 1));
+        s.sheet_cur.zero(st);
+        std::vector<uint32_t> fl(P->n_sheets);
+        for (uint64_t q = 0; q < P->n_sheets; q++) std::memcpy(&fl[q], P->sheets + 16 * q, 4);
+        std::sort(fl.begin(), fl.end());
+        const bool shared = std::adjacent_find(fl.begin(), fl.end()) != fl.end();
+        s.sheet = SheetArgs{P->n_sheets, s.sheets.p, s.sheet_cur.p, s.branches.p, shared ? 1u : 0u};
+    }
+    // thin wires: the table (split engine) and its per-plane list of H nodes (fused march)
+    if (P->n_wires) {
+        s.wires.upload(P->wires, P->n_wires, st);
+        std::vector<uint64_t> hs(P->n_wires);
+        for (uint64_t q = 0; q < P->n_wires; q++) hs[q] = P->wires[q].h;
+        std::sort(hs.begin(), hs.end());
+        s.wire_serial = std::adjacent_find(hs.begin(), hs.end()) != hs.end();
+        std::vector<std::vector<uint4>> by_plane(P->n[0]);
+        for (uint64_t q = 0; q < P->n_wires; q++) {
+            const uint64_t h = P->wires[q].h, c = h / nn, id = h % nn;
+            const uint32_t i = uint32_t(id / (uint64_t(P->n[1]) * P->n[2])), j = uint32_t((id / P->n[2]) % P->n[1]),
+                           k = uint32_t(id % P->n[2]);
+            by_plane[i].push_back(make_uint4(j, k, uint32_t(c), uint32_t(q)));
+        }
+        std::vector<uint32_t> start(P->n[0] + 1, 0);
+        std::vector<uint4> ent;
+        for (uint32_t i = 0; i < P->n[0]; i++) {
+            start[i] = uint32_t(ent.size());
+            ent.insert(ent.end(), by_plane[i].begin(), by_plane[i].end());
+        }
+        start[P->n[0]] = uint32_t(ent.size());
+        s.wire_start.upload(start.data(), start.size(), st);
+        s.wire_ent.upload(ent.data(), ent.size(), st);
+    }
     f.ind_i = s.ind_i.p;
     double t0, tau;
     tau = std::sqrt(std::log(10.0)) / (3.14159265358979323846 * pulse->fc);  // pulse_shape, sim.rs:646-649
@@ -481,6 +610,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     if (const char *v = getenv("FIELDGRID_SEG")) s.seg = std::max(1u, uint32_t(atoi(v)));
     if (const char *v = getenv("FIELDGRID_XCDS")) s.xcds = std::max(1u, uint32_t(atoi(v)));
     if (s.tj != 8 && s.tj != 14) s.tj = 14;
+    if (P->n_wires) s.tj = 14;  // the thin-wire march comes in one tile height
     s.seg = std::min<uint32_t>(s.seg, SEG_MAX);
     // a grid too small to give ~6 blocks per CU at `seg` planes per block marches shorter
     // segments instead (the results do not depend on the tiling)
@@ -512,7 +642,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     }
     const char *f2 = getenv("FIELDGRID_FUSE2");
     s.fuse2 = s.fused && !(f2 && f2[0] == '0') && !observed_debye && f.n_ind == 0 && !f.serial && o.plane_n == 0 &&
-              o.n_patches == 0;
+              o.n_patches == 0 && P->n_sheets == 0 && P->n_wires == 0;
     if (s.fuse2) {
         s.seg2 = 32;
         if (const char *v = getenv("FIELDGRID_SEG2")) s.seg2 = std::max(1u, uint32_t(atoi(v)));
@@ -569,6 +699,10 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
 
 void reset(Session &s) {
     hipStream_t st = s.stream;
+    if (s.sheet.n) {
+        FG_CHECK(hipMemcpyAsync(s.sheets.p, s.sheets0.data(), s.sheets0.size() * 4, hipMemcpyHostToDevice, st));
+        s.sheet_cur.zero(st);
+    }
     for (int c = 0; c < 2; c++) {
         s.E[c].zero(st);
         s.H[c].zero(st);
@@ -643,12 +777,32 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
             if (lds) FG_MARCH2(TJ, false, true); else FG_MARCH2(TJ, false, false); \
         }                                                              \
     } while (0)
-        switch (s.tj) {
-            case 8: FG_MARCH(8); break;
-            default: FG_MARCH(14); break;
+        if (s.wires.n) {
+            const WireArgs wa{s.wire_start.p, s.wire_ent.p, s.wires.p};
+#define FG_MARCHW(W, L)                                                                                            \
+    do {                                                                                                           \
+        if (ni) hipLaunchKernelGGL((k_march_wires<0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+        if (nz) hipLaunchKernelGGL((k_march_wires<1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+        if (np_) hipLaunchKernelGGL((k_march_wires<2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+    } while (0)
+            const bool lds = g.n_sets <= uint32_t(SETS_LDS);
+            if (g.wide) {
+                if (lds) FG_MARCHW(true, true); else FG_MARCHW(true, false);
+            } else {
+                if (lds) FG_MARCHW(false, true); else FG_MARCHW(false, false);
+            }
+#undef FG_MARCHW
+        } else {
+            switch (s.tj) {
+                case 8: FG_MARCH(8); break;
+                default: FG_MARCH(14); break;
+            }
         }
 #undef FG_MARCH2
 #undef FG_MARCH
+        if (s.sheet.n)
+            hipLaunchKernelGGL(k_sheet, dim3(s.sheet.serial ? 1u : uint32_t((s.sheet.n + 255) / 256)), dim3(256), 0, st,
+                               s.sheet, En, Eo, Hn);
         Ecur = s.E[b].p;
         Eold = s.E[a].p;
         s.cur = b;
@@ -656,10 +810,18 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
         dim3 block(64, 4);
         dim3 grid((g.n2 + 63) / 64, (g.n1 + 3) / 4, g.n0);
         hipLaunchKernelGGL(k_update_h, grid, block, 0, st, g, s.E[0].p, s.H[0].p, s.psi[0].p);
+        if (s.wires.n)
+            hipLaunchKernelGGL(k_wires, dim3(s.wire_serial ? 1u : uint32_t((s.wires.n + 255) / 256)), dim3(256), 0, st,
+                               s.wires.p, s.wires.n, s.wire_serial ? 1u : 0u, s.E[0].p, s.H[0].p);
+        const uint32_t sb = s.sheet.serial ? 1u : uint32_t((s.sheet.n + 255) / 256);
+        if (s.sheet.n)
+            hipLaunchKernelGGL(k_sheet_pre, dim3(uint32_t((s.sheet.n + 255) / 256)), dim3(256), 0, st, s.sheet, s.E[0].p,
+                               s.H[0].p);
         if (s.fix.n_debye)
             hipLaunchKernelGGL(k_debye_save, dim3(uint32_t((s.fix.n_debye + 255) / 256)), dim3(256), 0, st, s.fix.n_debye,
                                s.fix.debye, s.E[0].p, s.dold.p);
         hipLaunchKernelGGL(k_update_e, grid, block, 0, st, g, s.E[0].p, s.H[0].p, s.psi[0].p);
+        if (s.sheet.n) hipLaunchKernelGGL(k_sheet_post, dim3(sb), dim3(256), 0, st, s.sheet, s.E[0].p);
         Ecur = s.E[0].p;
     }
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));

@@ -111,6 +111,14 @@ struct Session {
     DevBuf<uint4> src2_ent;
     DevBuf<int32_t> deb2_row, deb2_map;  // Debye k maps of the two-step march (Deb2)
     DevBuf<float> deb2_jn, deb2_eo;
+    // resistive sheets (records restored from sheets0 on reset) and thin wires
+    SheetArgs sheet{};
+    DevBuf<float> sheets, sheet_cur, branches;
+    std::vector<float> sheets0;
+    DevBuf<fg_wire_h> wires;
+    DevBuf<uint32_t> wire_start;
+    DevBuf<uint4> wire_ent;
+    bool wire_serial = false;
     std::vector<std::pair<size_t, uint32_t>> units;  // timed launch units: (event slot, steps)
     std::vector<hipEvent_t> events;
     // host copies the batched sweep and reset() need

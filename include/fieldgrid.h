@@ -33,7 +33,7 @@ enum fg_status {
     FG_CANCELLED = 1,   /* progress returned 0: "cancelled" (run.rs:742-744)                    */
     FG_UNSTABLE = 2,    /* non-finite energy: "the FDTD run went unstable" (run.rs:736-738)     */
     FG_TOO_LARGE = 3,   /* "{cells} cells is more than this GPU can hold" (run.rs:401-403)      */
-    FG_UNSUPPORTED = 4, /* a feature the device engine does not run (caller falls back to CPU)  */
+    FG_UNSUPPORTED = 4, /* a feature an entry point does not run (caller falls back to CPU)    */
     FG_HIP = 5,         /* a HIP runtime error                                                 */
     FG_INVALID = 6      /* malformed input (bad index, size mismatch, NULL output buffer)       */
 };

@@ -57,7 +57,7 @@ typedef struct fg_spectra {
    `count` (an entry with nf = 0 is skipped). progress(user, steps, decay) is called on the
    calling thread whenever any run reaches a decay check, with the batch's step count and the
    smallest decay among the runs still going; returning 0 cancels the whole batch. Resistive
-   sheets and thin wires give FG_UNSUPPORTED, as in fg_run. A run that goes unstable fails the
+   sheets and thin wires give FG_UNSUPPORTED here (fg_run runs them). A run that goes unstable fails the
    batch with FG_UNSTABLE and "run <r>: the FDTD run went unstable". */
 int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
                  const fg_pulse *pulses, const fg_extras *const *extras, fg_tune *tune, fg_progress_fn progress,

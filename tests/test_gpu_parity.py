@@ -121,11 +121,44 @@ def test_fused_and_split_engines_agree(gpu):
 
 
 @pytest.mark.gpu
-def test_sheets_and_wires_are_refused_so_the_caller_can_fall_back(gpu):
+@pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
+def test_sheets_and_wires_run_on_the_device(gpu, engine):
+    """tests/fdtd.rs:43-174 with every feature, resistive sheets (two adaptive, the reference's
+    branch table) and thin wires included, both ports driven: bit-identical to the FP32 path.
+    The fused engine adds the wire terms inside the march (every copy of the H node) and runs the
+    sheets after it from the step's input E; the split engine runs them between its sweeps."""
     sim = everything()
-    ex = everything_extras(sim, 10)
+    assert sim.sheets == 6 and sim.wires > 30
+    for driven in (0, 1):
+        ex = everything_extras(sim, 600)
+        ex.engine = engine
+        rec = F.run(gpu, sim, driven, F.Pulse(5e9, 5e9), ex, fields=True)
+        _exact(rec, O.run_cpu(sim.desc, driven, F.Pulse(5e9, 5e9), ex, 32), f"everything with sheets, wires, driven {driven}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("radius", [0.09e-3, 1.7e-3])
+def test_thin_wires_at_either_end_of_their_radius_stay_stable_on_the_device(gpu, radius):
+    """tests/fdtd.rs:238-270 on the device (40 000 steps), and its first 3000 steps bit-identical
+    to the FP32 path."""
+    from .test_oracle_sheets_wires import thin_wire_case
+    sim, ex, pulse = thin_wire_case(radius)
+    rec = F.run(gpu, sim, 0, pulse, ex)
+    v = np.abs(np.asarray(rec.series).reshape(-1, 2)[:, 0])
+    early, late = v[5000:10000].max(), v[35000:40000].max()
+    print(f"\nradius {radius}: early {early:.3e} late {late:.3e}")
+    assert late < 2.0 * early
+    short, _, _ = thin_wire_case(radius, steps=3000)
+    ex = F.Extras(max_steps=3000, decay_db=math.inf)
+    _exact(F.run(gpu, short, 0, pulse, ex, fields=True), O.run_cpu(short.desc, 0, pulse, ex, 32), f"wire {radius}")
+
+
+@pytest.mark.gpu
+def test_a_batch_refuses_sheets_and_wires(gpu):
+    sim = everything()
+    job = F.BatchRun(sim, 0, F.Pulse(5e9, 5e9), F.Extras(max_steps=10, decay_db=math.inf), [])
     with pytest.raises(abi.FieldgridError) as e:
-        F.run(gpu, sim, 0, F.Pulse(5e9, 5e9), ex)
+        F.run_batch(gpu, [job])
     assert e.value.status == abi.FG_UNSUPPORTED
 
 
