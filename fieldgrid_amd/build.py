@@ -40,12 +40,14 @@ def build(verbose=True):
     sim = os.path.join(tmp, "sim.o")
     bat = os.path.join(tmp, "batch.o")
     adj = os.path.join(tmp, "adjoint.o")
+    far = os.path.join(tmp, "far.o")
     _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng])
     _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", bat])
     _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "adjoint.hip"), "-o", adj])
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "far.hip"), "-o", far])
     _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sim.cpp"), "-o", sim])
     out_tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, bat, adj, sim, "-o", out_tmp, "-lpthread"])
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, bat, adj, far, sim, "-o", out_tmp, "-lpthread"])
     os.replace(out_tmp, LIB)
     return LIB
 

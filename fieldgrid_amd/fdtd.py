@@ -787,43 +787,61 @@ def patches(sim: Sim, inset: int) -> Patches:
 
 
 class Far:
-    """ntff.rs:61-137: the f64 far-field transform of the NTFF accumulators."""
+    """ntff.rs:66-137 on the device (include/fieldgrid_far.h): the f64 far-field transform of
+    one frequency's NTFF accumulators. Needs a Gpu, like run()."""
 
-    def __init__(self, pos, area, j, m, freq):
-        self.pos, self.area, self.j, self.m, self.freq = pos, area, j, m, freq
+    def __init__(self, gpu: "Gpu", p: "Patches", acc: np.ndarray, nf: int, k: int, freq: float,
+                 scale=(1.0, 0.0)):
+        if gpu is None:
+            raise ValueError("Far needs a Gpu")
+        pos = np.ascontiguousarray(p.pos, np.float64)
+        area = np.ascontiguousarray(p.area, np.float64)
+        acc = np.ascontiguousarray(acc, np.float32)
+        count = len(area)
+        if acc.size < count * nf * 12:
+            raise ValueError("accumulators shorter than patches x nf x 12")
+        h = C.c_void_p()
+        dp = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        abi.check(abi.lib().fg_far_new(gpu._h, count, dp(pos, C.c_double), dp(area, C.c_double), dp(acc, C.c_float),
+                                       nf, k, float(freq), float(scale[0]), float(scale[1]), C.byref(h)))
+        self._h, self.freq, self.count = h, float(freq), count
 
     @staticmethod
-    def load(p: Patches, acc: np.ndarray, nf: int, k: int, freq: float, scale=(1.0, 0.0)) -> "Far":
-        count = len(p.pos)
-        a = np.asarray(acc, np.float64).reshape(count, nf, 12)[:, k, :]
-        z = a[:, 0::2] + 1j * a[:, 1::2]
-        z = z * complex(scale[0], scale[1])
-        return Far(p.pos, p.area, z[:, :3], z[:, 3:], freq)
+    def load(gpu: "Gpu", p: "Patches", acc: np.ndarray, nf: int, k: int, freq: float, scale=(1.0, 0.0)) -> "Far":
+        """Far::load (ntff.rs:76-88)."""
+        return Far(gpu, p, acc, nf, k, freq, scale)
+
+    @staticmethod
+    def _dirs(d):
+        d = np.ascontiguousarray(np.asarray(d, np.float64).reshape(-1, 3))
+        return d, d.ctypes.data_as(C.POINTER(C.c_double))
 
     def field(self, d) -> np.ndarray:
-        d = np.asarray(d, np.float64)
-        k = 2.0 * math.pi * self.freq / C0
-        arg = k * (self.pos @ d)
-        ph = self.area * np.exp(1j * arg)
-        nn_ = (self.j * ph[:, None]).sum(0)
-        ll = (self.m * ph[:, None]).sum(0)
-        rn = (nn_ * d).sum()
-        perp = nn_ - rn * d
-        rl = np.cross(d, ll)
-        return (rl - perp * ETA0) * (k / (4.0 * math.pi))
+        """Far::field (ntff.rs:90-115): one direction -> (3,) complex, or (n, 3) -> (n, 3)."""
+        single = np.asarray(d).ndim == 1
+        d, dp = self._dirs(d)
+        out = np.zeros((len(d), 6))
+        abi.check(abi.lib().fg_far_field(self._h, len(d), dp, out.ctypes.data_as(C.POINTER(C.c_double))))
+        z = out[:, 0::2] + 1j * out[:, 1::2]
+        return z[0] if single else z
 
-    def intensity(self, d) -> float:
-        f = self.field(d)
-        return float((f.real ** 2 + f.imag ** 2).sum() / (2.0 * ETA0))
+    def intensity(self, d):
+        """Far::intensity (ntff.rs:117-119): a float for one direction, else an array."""
+        single = np.asarray(d).ndim == 1
+        d, dp = self._dirs(d)
+        out = np.zeros(len(d))
+        abi.check(abi.lib().fg_far_intensity(self._h, len(d), dp, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return float(out[0]) if single else out
 
     def radiated(self, steps: int) -> Tuple[float, float]:
-        total, peak = 0.0, 0.0
-        for a in range(steps):
-            th = math.pi * (a + 0.5) / steps
-            for b in range(2 * steps):
-                ph = math.pi * b / steps
-                d = [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]
-                u = self.intensity(d)
-                peak = max(peak, u)
-                total += u * math.sin(th) * (math.pi / steps) ** 2
-        return total, peak
+        """Far::radiated (ntff.rs:121-136): (total power, peak intensity)."""
+        t, pk = C.c_double(), C.c_double()
+        abi.check(abi.lib().fg_far_radiated(self._h, steps, C.byref(t), C.byref(pk)))
+        return t.value, pk.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            abi.lib().fg_far_free(self._h)
+            self._h = None
+
+    __del__ = close

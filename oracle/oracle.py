@@ -217,3 +217,46 @@ def design_gradient(desc, driven, pulse, steps, obs, lo, hi, dseries):
     if st != 0:
         raise OracleError(st, lib().oracle_last_error().decode())
     return grad, series[:, :n_obs]
+
+
+class Far:
+    """ntff.rs:66-137 restated in numpy f64 (Far::load, field, intensity, radiated): the checker
+    of the device far field."""
+
+    def __init__(self, pos, area, j, m, freq):
+        self.pos, self.area, self.j, self.m, self.freq = pos, area, j, m, freq
+
+    @staticmethod
+    def load(p, acc, nf, k, freq, scale=(1.0, 0.0)):
+        """ntff.rs:76-88"""
+        count = len(p.pos)
+        a = np.asarray(acc, np.float64).reshape(count, nf, 12)[:, k, :]
+        z = (a[:, 0::2] + 1j * a[:, 1::2]) * complex(scale[0], scale[1])
+        return Far(np.asarray(p.pos, np.float64), np.asarray(p.area, np.float64), z[:, :3], z[:, 3:], freq)
+
+    def field(self, d):
+        """ntff.rs:90-115"""
+        d = np.asarray(d, np.float64)
+        k = 2.0 * np.pi * self.freq / 299_792_458.0
+        ph = self.area * np.exp(1j * k * (self.pos @ d))
+        nn = (self.j * ph[:, None]).sum(0)
+        ll = (self.m * ph[:, None]).sum(0)
+        perp = nn - (nn * d).sum() * d
+        return (np.cross(d, ll) - perp * 376.730_313_668) * (k / (4.0 * np.pi))
+
+    def intensity(self, d):
+        """ntff.rs:117-119"""
+        f = self.field(d)
+        return float((f.real ** 2 + f.imag ** 2).sum() / (2.0 * 376.730_313_668))
+
+    def radiated(self, steps):
+        """ntff.rs:121-136"""
+        total, peak = 0.0, 0.0
+        for a in range(steps):
+            th = np.pi * (a + 0.5) / steps
+            for b in range(2 * steps):
+                ph = np.pi * b / steps
+                u = self.intensity([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+                peak = max(peak, u)
+                total += u * np.sin(th) * (np.pi / steps) * (np.pi / steps)
+        return total, peak

@@ -220,7 +220,7 @@ def dipole_power(nodes=56, pml=8, inset=3, half=6, f=6e9, steps=2500):
     V = np.sum(rec.series[0::2] * np.exp(-1j * w * (s + 1.0) * dt)) * dt
     I = np.sum(rec.series[1::2] * np.exp(-1j * w * (s + 0.5) * dt)) * dt
     p_in = 0.5 * (V * np.conj(I)).real
-    far = F.Far.load(pt, rec.ntff, 1, 0, f)
+    far = O.Far.load(pt, rec.ntff, 1, 0, f)
     total, peak = radiated(far, 36)
     return total / p_in, 4 * math.pi * peak / total
 
