@@ -655,3 +655,45 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         return int(FG_OK);
     });
 }
+
+// finish()'s spectra (crates/sims-antenna/src/fdtd/mod.rs:229-249) of one record's series, on
+// the device: k_spectra over every (port, frequency).
+extern "C" int fg_port_spectra(fg_device *dev, const float *series, uint64_t steps, uint32_t ports, double dt,
+                               uint32_t nf, const double *freqs, double *v, double *i) {
+    using namespace fg;
+    return guarded([&] {
+        if (!dev || (steps && ports && !series) || (nf && (!freqs || !v || !i)))
+            throw HipError{"fg_port_spectra: NULL argument", FG_INVALID};
+        const uint64_t n = uint64_t(ports) * nf;
+        if (!n) return int(FG_OK);
+        if (n > 0x7fffff00ull) throw HipError{"fg_port_spectra: too many ports x frequencies", FG_INVALID};
+        FG_CHECK(hipSetDevice(dev->ordinal));
+        hipStream_t st = nullptr;
+        FG_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s); }
+        } guard{st};
+        DevBuf<float> ds;
+        if (steps) ds.upload(series, steps * ports * 2, st);
+        else ds.alloc(1);
+        std::vector<SpecJob> jobs;
+        for (uint32_t p = 0; p < ports; p++)
+            for (uint32_t f = 0; f < nf; f++)
+                jobs.push_back(SpecJob{ds.p, steps, ports, p, dt, 2.0 * 3.14159265358979323846 * freqs[f]});
+        DevBuf<SpecJob> dj;
+        DevBuf<double> out;
+        dj.upload(jobs.data(), jobs.size(), st);
+        out.alloc(4 * jobs.size());
+        hipLaunchKernelGGL(k_spectra, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, dj.p, uint32_t(n), out.p);
+        FG_CHECK(hipGetLastError());
+        std::vector<double> h(4 * n);
+        FG_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, st));
+        FG_CHECK(hipStreamSynchronize(st));
+        for (uint64_t q = 0; q < n; q++) {
+            v[2 * q] = h[4 * q], v[2 * q + 1] = h[4 * q + 1];
+            i[2 * q] = h[4 * q + 2], i[2 * q + 1] = h[4 * q + 3];
+        }
+        return int(FG_OK);
+    });
+}

@@ -708,16 +708,44 @@ class Design:
         self.close()
 
 
-def port_spectra(series: np.ndarray, ports: int, dt: float, freqs: Sequence[float]):
+def port_spectra(gpu: Gpu, series: np.ndarray, ports: int, dt: float, freqs: Sequence[float]):
     """The sims-antenna port spectra (crates/sims-antenna/src/fdtd/mod.rs:231-246) of a record's
-    series, per port, in f64 on the host: V(f) = sum_n V_n dt e^{-iw(n+1)dt},
-    I(f) = sum_n I_n dt e^{-iw(n+1/2)dt}. Returns (V, I) of shape (ports, len(freqs))."""
-    s = np.asarray(series, np.float64).reshape(-1, ports, 2)
-    n = np.arange(s.shape[0], dtype=np.float64)
-    w = 2.0 * np.pi * np.asarray(freqs, np.float64)
-    pe = dt * np.exp(-1j * np.outer(w, n + 1.0) * dt)
-    ph = dt * np.exp(-1j * np.outer(w, n + 0.5) * dt)
-    return (pe @ s[:, :, 0]).T, (ph @ s[:, :, 1]).T
+    series, per port, on the device (fg_port_spectra): V(f) = sum_n V_n dt e^{-iw(n+1)dt},
+    I(f) = sum_n I_n dt e^{-iw(n+1/2)dt} by the caller's f64 phasor recursion. Returns (V, I)
+    of shape (ports, len(freqs))."""
+    if gpu is None:
+        raise ValueError("port_spectra() needs a Gpu")
+    s = np.ascontiguousarray(series, np.float32)
+    steps = s.size // max(2 * ports, 1)
+    f = np.ascontiguousarray(freqs, np.float64)
+    v = np.zeros(max(ports * len(f) * 2, 1))
+    i = np.zeros_like(v)
+    dp = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    abi.check(abi.lib().fg_port_spectra(gpu._h, dp(s, C.c_float), steps, ports, float(dt), len(f), dp(f, C.c_double),
+                                        dp(v, C.c_double), dp(i, C.c_double)))
+    shape = (ports, len(f))
+    V = (v[0::2] + 1j * v[1::2])[: ports * len(f)].reshape(shape)
+    I = (i[0::2] + 1j * i[1::2])[: ports * len(f)].reshape(shape)
+    return V, I
+
+
+@dataclass
+class Outcome:
+    """What sims-antenna's finish() (crates/sims-antenna/src/fdtd/mod.rs:229-266) derives from a
+    single-port record: the impedance sweep, the impedance and input power at f0."""
+    freqs: List[float]
+    z: np.ndarray
+    z0: complex
+    p_in: float
+
+
+def finish(gpu: Gpu, rec: Record, dt: float, freqs: Sequence[float], f0: float) -> Outcome:
+    """finish() (mod.rs:229-266) for port 0 of `rec`: spectra at freqs and f0 on the device,
+    Z = V / I, p_in = Re(V(f0) I(f0)*) / 2."""
+    allf = list(freqs) + [f0]
+    V, I = port_spectra(gpu, rec.series, rec.ports, dt, allf)
+    z = V[0] / I[0]
+    return Outcome(list(freqs), z[:-1], complex(z[-1]), float(0.5 * (V[0, -1] * np.conj(I[0, -1])).real))
 
 
 class Session:

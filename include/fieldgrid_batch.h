@@ -63,6 +63,13 @@ int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *proble
                  const fg_pulse *pulses, const fg_extras *const *extras, fg_tune *tune, fg_progress_fn progress,
                  void *user, fg_record *records, fg_spectra *spectra);
 
+/* finish()'s spectra of one record (crates/sims-antenna/src/fdtd/mod.rs:229-249) on the device,
+   as fg_spectra computes them for a batch: series as fg_record.series (steps * ports * 2
+   floats), v and i as in fg_spectra ([(port*nf + f)*2 + {re, im}]). The caller forms Z = V/I and
+   p_in = Re(V I*)/2 as finish() does. */
+int fg_port_spectra(fg_device *dev, const float *series, uint64_t steps, uint32_t ports, double dt, uint32_t nf,
+                    const double *freqs, double *v, double *i);
+
 #ifdef __cplusplus
 }
 #endif

@@ -260,3 +260,15 @@ class Far:
                 peak = max(peak, u)
                 total += u * np.sin(th) * (np.pi / steps) * (np.pi / steps)
         return total, peak
+
+
+def port_spectra(series, ports, dt, freqs):
+    """The sims-antenna port spectra (crates/sims-antenna/src/fdtd/mod.rs:231-246) by direct
+    exponentials in f64: V(f) = sum_n V_n dt e^{-iw(n+1)dt}, I(f) = sum_n I_n dt e^{-iw(n+1/2)dt}.
+    Returns (V, I) of shape (ports, len(freqs))."""
+    s = np.asarray(series, np.float64).reshape(-1, ports, 2)
+    n = np.arange(s.shape[0], dtype=np.float64)
+    w = 2.0 * np.pi * np.asarray(freqs, np.float64)
+    pe = dt * np.exp(-1j * np.outer(w, n + 1.0) * dt)
+    ph = dt * np.exp(-1j * np.outer(w, n + 0.5) * dt)
+    return (pe @ s[:, :, 0]).T, (ph @ s[:, :, 1]).T

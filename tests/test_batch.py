@@ -16,6 +16,7 @@ import pytest
 
 from fieldgrid_amd import fdtd as F
 from fieldgrid_amd import workloads as W
+from oracle import oracle as O
 
 from .fixtures import worst
 
@@ -42,13 +43,13 @@ def finish_spectra(series, ports, dt, freqs):
 
 
 def test_port_spectra_match_the_callers_phasor_recursion():
-    """fdtd.port_spectra (direct exponentials) against the recursion of the sims-antenna caller,
-    and both against the closed form for a unit impulse at step n0: V = dt e^{-iw(n0+1)dt}."""
+    """The oracle's port spectra (direct exponentials) against the recursion of the sims-antenna
+    caller, and both against the closed form for a unit impulse at step n0: V = dt e^{-iw(n0+1)dt}."""
     rng = np.random.default_rng(7)
     ports, steps, dt = 3, 257, 1.3e-12
     freqs = [1e9, 7.5e9, 19e9]
     series = rng.standard_normal(steps * ports * 2).astype(np.float32)
-    V, I = F.port_spectra(series, ports, dt, freqs)
+    V, I = O.port_spectra(series, ports, dt, freqs)
     V2, I2 = finish_spectra(series, ports, dt, freqs)
     assert np.max(np.abs(V - V2)) <= 1e-9 * np.max(np.abs(V2))
     assert np.max(np.abs(I - I2)) <= 1e-9 * np.max(np.abs(I2))
@@ -56,7 +57,7 @@ def test_port_spectra_match_the_callers_phasor_recursion():
     n0 = 40
     imp[n0, 1, 0] = 1.0
     imp[n0, 1, 1] = 1.0
-    V, I = F.port_spectra(imp.ravel(), ports, dt, freqs)
+    V, I = O.port_spectra(imp.ravel(), ports, dt, freqs)
     for q, f in enumerate(freqs):
         w = 2 * math.pi * f
         assert abs(V[1, q] - dt * cmath.exp(-1j * w * (n0 + 1) * dt)) < 1e-12 * dt
@@ -109,14 +110,13 @@ def test_batched_runs_equal_single_runs(gpu, tune):
         single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
         _same(recs[q], single, f"run {q}")
         V, I = spectra[q]
-        V2, I2 = F.port_spectra(single.series, len(r.sim.ports), r.sim.dt, r.freqs)
+        V2, I2 = O.port_spectra(single.series, len(r.sim.ports), r.sim.dt, r.freqs)
         assert np.max(np.abs(V - V2)) <= 1e-9 * max(np.max(np.abs(V2)), 1e-300)
         assert np.max(np.abs(I - I2)) <= 1e-9 * max(np.max(np.abs(I2)), 1e-300)
 
 
 @pytest.mark.gpu
 def test_a_batched_run_matches_the_oracle(gpu):
-    from oracle import oracle as O
     runs = small_sweep(count=3, steps=(120, 160))
     recs, _ = F.run_batch(gpu, runs, tune=F.Tune(chunk=8))
     for q, r in enumerate(runs):
@@ -165,3 +165,24 @@ def test_progress_can_cancel_a_batch(gpu):
         F.run_batch(gpu, runs, tune=F.Tune(tj=14, seg=16, chunk=8),
                     progress=lambda s, d: seen.append(s) or s < 200)
     assert str(e.value) == "cancelled" and seen == [100, 200]
+
+
+@pytest.mark.gpu
+def test_device_port_spectra_and_finish_match_the_direct_sums(gpu):
+    """fg_port_spectra (finish()'s recursion on the device) against the oracle's direct sums, and
+    finish()'s Z and p_in from them (crates/sims-antenna/src/fdtd/mod.rs:229-266)."""
+    rng = np.random.default_rng(9)
+    ports, steps, dt = 2, 3001, 1.1e-12
+    freqs = list(np.linspace(1e9, 20e9, 64))
+    series = rng.standard_normal(steps * ports * 2).astype(np.float32)
+    V, I = F.port_spectra(gpu, series, ports, dt, freqs)
+    V2, I2 = O.port_spectra(series, ports, dt, freqs)
+    assert np.max(np.abs(V - V2)) <= 1e-9 * np.max(np.abs(V2))
+    assert np.max(np.abs(I - I2)) <= 1e-9 * np.max(np.abs(I2))
+    rec = F.Record(steps, ports, series, 0.0, np.zeros(0), np.zeros(0))
+    out = F.finish(gpu, rec, dt, freqs[:10], 7e9)
+    V3, I3 = O.port_spectra(series, ports, dt, freqs[:10] + [7e9])
+    z = V3[0] / I3[0]
+    assert np.max(np.abs(out.z - z[:-1])) <= 1e-9 * np.max(np.abs(z))
+    assert abs(out.z0 - z[-1]) <= 1e-9 * abs(z[-1])
+    assert abs(out.p_in - 0.5 * (V3[0, -1] * np.conj(I3[0, -1])).real) <= 1e-9 * abs(V3[0, -1] * I3[0, -1])
