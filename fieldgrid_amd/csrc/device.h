@@ -894,46 +894,64 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
     // step-n psi of the previous plane (component, axis)
     float q_hxy = 0, q_hxz = 0, q_hyz = 0, q_hyx = 0, q_hzx = 0, q_hzy = 0;
     float q_exy = 0, q_exz = 0, q_eyz = 0, q_eyx = 0, q_ezx = 0, q_ezy = 0;
-    for (uint32_t t = tstart; t <= i1; t++) {
-        const int b = t & 1;
+    // the loads of plane t: E0(t+1), H0(t), coefficient indices and step n-1 psi. They are issued
+    // one iteration ahead (plane t+1's while plane t computes), so their latency overlaps a
+    // whole iteration instead of stalling between the barriers.
+    struct Pl {
+        float exn, eyn, ezn, hx0, hy0, hz0;
+        uint32_t c0, c1, c2;
+        float hxy, hxz, hyz, hyx, hzx, hzy, exy, exz, eyz, eyx, ezx, ezy;
+    };
+    auto load_plane = [&](uint32_t t, Pl &P) {
         const uint64_t pb = uint64_t(t) * plane;
-        const bool il = t + 1 < g.n0;
-        // loads of plane t: E0(t+1), H0(t), coefficient indices, step n-1 psi
         const float *pe = pbase(Eo, int64_t(t) + 1);
         const uint32_t eb = pbytes(int64_t(t) + 1, pl4);
-        const float exn = ld(pe, ldA, eb), eyn = ld(pe + nn, ldA, eb), ezn = ld(pe + 2 * nn, ldA, eb);
+        P.exn = ld(pe, ldA, eb), P.eyn = ld(pe + nn, ldA, eb), P.ezn = ld(pe + 2 * nn, ldA, eb);
         const float *ph = pbase(Ho, t);
         const uint32_t hb = pbytes(t, pl4);
-        const float hx0 = ld(ph, ldA, hb), hy0 = ld(ph + nn, ldA, hb), hz0 = ld(ph + 2 * nn, ldA, hb);
+        P.hx0 = ld(ph, ldA, hb), P.hy0 = ld(ph + nn, ldA, hb), P.hz0 = ld(ph + 2 * nn, ldA, hb);
         const uint32_t cbytes = pbytes(t, plc);
         const uint64_t cb_ = t < g.n0 ? pb : 0;
-        const uint32_t c0 = cidx(cb_, cbytes), c1 = cidx(nn + cb_, cbytes), c2 = cidx(2 * nn + cb_, cbytes);
+        P.c0 = cidx(cb_, cbytes), P.c1 = cidx(nn + cb_, cbytes), P.c2 = cidx(2 * nn + cb_, cbytes);
+        P.hxy = P.hxz = P.hyz = P.hyx = P.hzx = P.hzy = 0.f;
+        P.exy = P.exz = P.eyz = P.eyx = P.ezx = P.ezy = 0.f;
+        const uint32_t zb = t < g.n0 ? zbytes : 0u, yb = t < g.n0 ? ybytes : 0u, xb = t < g.n0 ? pl4 : 0u;
+        if constexpr (PZ) {
+            const uint64_t zrow = t < g.n0 ? uint64_t(t) * g.n1 * p2 : 0;
+            P.hxz = ld(psio + PSI(1, 0, 2) + zrow, ozh, zb), P.hyz = ld(psio + PSI(1, 1, 2) + zrow, ozh, zb);
+            P.exz = ld(psio + PSI(0, 0, 2) + zrow, oze, zb), P.eyz = ld(psio + PSI(0, 1, 2) + zrow, oze, zb);
+        }
+        if constexpr (PXY) {
+            const float *xv = xa[t - tstart];
+            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
+            const uint64_t yrow = t < g.n0 ? uint64_t(t) * p2 * g.n2 : 0;
+            const uint64_t xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
+            const uint32_t oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
+            P.hxy = ld(psio + PSI(1, 0, 1) + yrow, oyh, yb);
+            P.hyx = ld(psio + PSI(1, 1, 0) + xhb, oxh, xb);
+            P.hzx = ld(psio + PSI(1, 2, 0) + xhb, oxh, xb), P.hzy = ld(psio + PSI(1, 2, 1) + yrow, oyh, yb);
+            P.exy = ld(psio + PSI(0, 0, 1) + yrow, oye, yb);
+            P.eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, xb);
+            P.ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, xb), P.ezy = ld(psio + PSI(0, 2, 1) + yrow, oye, yb);
+        }
+    };
+    Pl nxt;
+    load_plane(tstart, nxt);
+    for (uint32_t t = tstart; t <= i1; t++) {
+        const int b = t & 1;
+        const bool il = t + 1 < g.n0;
+        const Pl cur = nxt;
+        if (t < i1) load_plane(t + 1, nxt);  // uniform
+        const float exn = cur.exn, eyn = cur.eyn, ezn = cur.ezn, hx0 = cur.hx0, hy0 = cur.hy0, hz0 = cur.hz0;
+        const uint32_t c0 = cur.c0, c1 = cur.c1, c2 = cur.c2;
         const float *xv = xa[t - tstart];
         const float idx_ = xv[0], iddx = xv[1];
         float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
-        float hxy = 0, hxz = 0, hyz = 0, hyx = 0, hzx = 0, hzy = 0;
-        float exy = 0, exz = 0, eyz = 0, eyx = 0, ezx = 0, ezy = 0;
-        uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
-        uint32_t oxh = OOB, oxe = OOB;
-        const uint32_t zb = t < g.n0 ? zbytes : 0u, yb = t < g.n0 ? ybytes : 0u, xb = t < g.n0 ? pl4 : 0u;
-        if constexpr (PZ) {
-            zrow = t < g.n0 ? uint64_t(t) * g.n1 * p2 : 0;
-            hxz = ld(psio + PSI(1, 0, 2) + zrow, ozh, zb), hyz = ld(psio + PSI(1, 1, 2) + zrow, ozh, zb);
-            exz = ld(psio + PSI(0, 0, 2) + zrow, oze, zb), eyz = ld(psio + PSI(0, 1, 2) + zrow, oze, zb);
-        }
+        float hxy = cur.hxy, hxz = cur.hxz, hyz = cur.hyz, hyx = cur.hyx, hzx = cur.hzx, hzy = cur.hzy;
+        float exy = cur.exy, exz = cur.exz, eyz = cur.eyz, eyx = cur.eyx, ezx = cur.ezx, ezy = cur.ezy;
         if constexpr (PXY) {
-            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
             bhx = xv[2], chx = xv[3], rdx = xv[4];
             bex = xv[5], cex = xv[6], rddx = xv[7];
-            yrow = t < g.n0 ? uint64_t(t) * p2 * g.n2 : 0;
-            xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
-            oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
-            hxy = ld(psio + PSI(1, 0, 1) + yrow, oyh, yb);
-            hyx = ld(psio + PSI(1, 1, 0) + xhb, oxh, xb);
-            hzx = ld(psio + PSI(1, 2, 0) + xhb, oxh, xb), hzy = ld(psio + PSI(1, 2, 1) + yrow, oyh, yb);
-            exy = ld(psio + PSI(0, 0, 1) + yrow, oye, yb);
-            eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, xb);
-            ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, xb), ezy = ld(psio + PSI(0, 2, 1) + yrow, oye, yb);
         }
         // ---- step n, H at plane t (the +1 neighbours of E0: j+1 from LDS, k+1 from the next lane)
         const float ex_j = xr[0][b ^ 1][w + 2][0][lane], ez_j = xr[0][b ^ 1][w + 2][1][lane];
