@@ -324,6 +324,19 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
         }
     __syncthreads();  // sets and xa (slot numbers become buffer bases) are read from here on
 #define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    // Loads run one plane ahead of their use, so their latency overlaps a whole iteration: at
+    // iteration i the raw values of plane i+2 (lamE, coefficient-set indices, E-psi adjoints) and
+    // of plane i+1 (lamH, H-psi adjoints) are issued, while plane i+1's terms are formed from the
+    // raw values issued the iteration before.
+    struct RawE {
+        float lx, ly, lz;
+        uint32_t c0, c1, c2;
+        float pxy, pxz, pyz, pyx, pzx, pzy;  // E-psi adjoints (component, axis)
+    };
+    struct RawH {
+        float hx, hy, hz;
+        float qxy, qxz, qyz, qyx, qzx, qzy;  // H-psi adjoints
+    };
     // the hv/hu terms of plane p from lamE, (ca, cb) and the E-psi adjoints of that plane
     struct Ed {
         float lx, ly, lz;                          // lamE
@@ -331,14 +344,14 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
         float pxy, pxz, pyz, pyx, pzx, pzy;        // E-psi adjoints (component, axis)
         float hv0, hu0, hv1, hu1, hv2, hu2;        // terms
     };
-    auto load_e = [&](uint32_t p, Ed &d) {
+    auto load_e = [&](uint32_t p, RawE &d) {
         const bool in_ = p < g.n0;
         const float *q = Lo + (in_ ? uint64_t(p) * plane : 0);
         const uint32_t b = in_ ? pl4 : 0u;
         d.lx = ld(q, ldA, b), d.ly = ld(q + nn, ldA, b), d.lz = ld(q + 2 * nn, ldA, b);
         const uint64_t cb_ = in_ ? uint64_t(p) * plane : 0;
         const uint32_t cbytes = in_ ? plc : 0u;
-        d.f0 = coef(cidx(cb_, cbytes)), d.f1 = coef(cidx(nn + cb_, cbytes)), d.f2 = coef(cidx(2 * nn + cb_, cbytes));
+        d.c0 = cidx(cb_, cbytes), d.c1 = cidx(nn + cb_, cbytes), d.c2 = cidx(2 * nn + cb_, cbytes);
         d.pxy = d.pxz = d.pyz = d.pyx = d.pzx = d.pzy = 0.f;
         if constexpr (PZ) {
             const uint64_t zrow = in_ ? uint64_t(p) * g.n1 * p2 : 0;
@@ -355,7 +368,28 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
             d.pzx = ld(po + PSI(0, 2, 0) + xeb, oxe, xb), d.pzy = ld(po + PSI(0, 2, 1) + yrow, oye, yb);
         }
     };
-    auto terms_e = [&](uint32_t p, Ed &d) {
+    auto load_h = [&](uint32_t i, RawH &d) {
+        const uint64_t pb = uint64_t(i) * plane;
+        d.hx = ld(Mo + pb, ldA, pl4), d.hy = ld(Mo + pb + nn, ldA, pl4), d.hz = ld(Mo + pb + 2 * nn, ldA, pl4);
+        d.qxy = d.qxz = d.qyz = d.qyx = d.qzx = d.qzy = 0.f;
+        if constexpr (PZ) {
+            const uint64_t zrow = uint64_t(i) * g.n1 * p2;
+            d.qxz = ld(po + PSI(1, 0, 2) + zrow, ozh, zbytes), d.qyz = ld(po + PSI(1, 1, 2) + zrow, ozh, zbytes);
+        }
+        if constexpr (PXY) {
+            const float *xs = xa[i - istart];
+            const int shx = __float_as_int(xs[2]);
+            const uint64_t yrow = uint64_t(i) * p2 * g.n2, xhb = uint64_t(max(shx, 0)) * plane;
+            const uint32_t oxh = shx >= 0 ? ldA : OOB;
+            d.qxy = ld(po + PSI(1, 0, 1) + yrow, oyh, ybytes);
+            d.qyx = ld(po + PSI(1, 1, 0) + xhb, oxh, pl4);
+            d.qzx = ld(po + PSI(1, 2, 0) + xhb, oxh, pl4), d.qzy = ld(po + PSI(1, 2, 1) + yrow, oyh, ybytes);
+        }
+    };
+    auto terms_e = [&](uint32_t p, const RawE &r, Ed &d) {
+        d.lx = r.lx, d.ly = r.ly, d.lz = r.lz;
+        d.f0 = coef(r.c0), d.f1 = coef(r.c1), d.f2 = coef(r.c2);
+        d.pxy = r.pxy, d.pxz = r.pxz, d.pyz = r.pyz, d.pyx = r.pyx, d.pzx = r.pzx, d.pzy = r.pzy;
         const float *xs = xa[min(p, i1) - istart];
         const float iddx = xs[1];
         const float ax_ = d.f0.y != 0.f ? d.f0.y * d.lx : 0.f, ay_ = d.f1.y != 0.f ? d.f1.y * d.ly : 0.f,
@@ -376,8 +410,12 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
         }
     };
     Ed cur, nxt;
-    load_e(istart, cur);
-    terms_e(istart, cur);
+    RawE re;
+    RawH rh;
+    load_e(istart, re);
+    terms_e(istart, re, cur);
+    load_e(istart + 1, re);  // istart < i1: the block has at least one plane
+    load_h(istart, rh);
     xv[istart & 1][w + 1][0][lane] = cur.hv0, xv[istart & 1][w + 1][1][lane] = cur.hu2;
     __syncthreads();
     float evz_p = 0.f, euy_p = 0.f;  // ev of Hz, eu of Hy at the plane before (the -x terms)
@@ -389,27 +427,26 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
         const uint32_t sto = wplane ? stO : OOB;
         const float *xs = xa[i - istart];
         const float idx_ = xs[0];
-        load_e(i + 1, nxt);
-        const float hx0 = ld(Mo + pb, ldA, pl4), hy0 = ld(Mo + pb + nn, ldA, pl4), hz0 = ld(Mo + pb + 2 * nn, ldA, pl4);
-        float qxy = 0, qxz = 0, qyz = 0, qyx = 0, qzx = 0, qzy = 0;  // H-psi adjoints of plane i
+        const RawE ne = re;  // plane i+1, issued the iteration before
+        const RawH h = rh;   // plane i
+        terms_e(i + 1, ne, nxt);
+        if (i + 1 < i1) {  // uniform
+            load_e(i + 2, re);
+            load_h(i + 1, rh);
+        }
+        const float hx0 = h.hx, hy0 = h.hy, hz0 = h.hz;
+        const float qxy = h.qxy, qxz = h.qxz, qyz = h.qyz, qyx = h.qyx, qzx = h.qzx, qzy = h.qzy;
         uint64_t zrow = 0, yrow = 0, xhb = 0, xeb = 0;
         uint32_t oxh = OOB, oxe = OOB;
         float bhx = 0, chx = 0, rdx = 0, bex = 0;
-        if constexpr (PZ) {
-            zrow = uint64_t(i) * g.n1 * p2;
-            qxz = ld(po + PSI(1, 0, 2) + zrow, ozh, zbytes), qyz = ld(po + PSI(1, 1, 2) + zrow, ozh, zbytes);
-        }
+        if constexpr (PZ) zrow = uint64_t(i) * g.n1 * p2;
         if constexpr (PXY) {
             const int shx = __float_as_int(xs[2]), sex = __float_as_int(xs[3]);
             bhx = xs[4], chx = xs[5], rdx = xs[6], bex = xs[7];
             yrow = uint64_t(i) * p2 * g.n2;
             xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
             oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
-            qxy = ld(po + PSI(1, 0, 1) + yrow, oyh, ybytes);
-            qyx = ld(po + PSI(1, 1, 0) + xhb, oxh, pl4);
-            qzx = ld(po + PSI(1, 2, 0) + xhb, oxh, pl4), qzy = ld(po + PSI(1, 2, 1) + yrow, oyh, ybytes);
         }
-        terms_e(i + 1, nxt);
         // lamH: + terms at the node, - terms at +z (next lane), +y (row below, LDS), +x (next plane)
         const float x1z = __shfl_down(cur.hv1, 1, 64), x0z = __shfl_down(cur.hu0, 1, 64);
         const float x0y = xv[b][w + 2][0][lane], x2y = xv[b][w + 2][1][lane];
