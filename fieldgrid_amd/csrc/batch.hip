@@ -75,7 +75,7 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun 
     const uint32_t n = r.fix.step[0] + s_off;
     const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
     march2<TJ2, PML, WIDE, LDS_SETS, true>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
-                                     r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db);
+                                     r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{});
 }
 
 __global__ __launch_bounds__(256) void k_fix_batch(const BatchRun *__restrict__ runs, const uint32_t *__restrict__ list,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_observe_mid_batch(const BatchRun *__res
                                                            const uint32_t *__restrict__ list, uint32_t par,
                                                            uint32_t s_off) {
     const BatchRun &r = runs[list[blockIdx.y]];
-    observe_mid(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], s_off, blockIdx.x);
+    observe_mid(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], s_off, blockIdx.x, Cap2{});
 }
 
 __global__ __launch_bounds__(256) void k_observe_batch(const BatchRun *__restrict__ runs,
@@ -597,7 +597,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             t.db = Deb2{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
             t.fix2 = t.fix;
             if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
-            if (!s.fuse2) all_pairs = false;
+            if (!s.fuse2 || s.cap_start.n) all_pairs = false;  // the batch hands observers no captures
         }
         b.pairs_ok = all_pairs && count > 0;
         b.table.upload(table.data(), table.size(), b.stream);

@@ -663,6 +663,63 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// One plane-DFT point q at step n (cpu.rs:336-361): f = {Ex, Ey, Ez, Hx, Hy} of the point.
+__device__ __forceinline__ void plane_acc(const ObsArgs &a, uint32_t q, uint32_t n, const float f[5]) {
+    const float dt = a.dt;
+    const float te = (float(n) + 1.0f) * dt, th = (float(n) + 0.5f) * dt;
+    float *out = a.plane + uint64_t(q) * a.nf * 10;
+    for (uint32_t kf = 0; kf < a.nf; kf++) {
+        const float w = a.omega[kf];
+        const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
+        const bool tab = n < a.n_phase;
+        const float ce0 = tab ? ph[0] : cosf(w * te) * dt, ce1 = tab ? ph[1] : -sinf(w * te) * dt;
+        const float ch0 = tab ? ph[2] : cosf(w * th) * dt, ch1 = tab ? ph[3] : -sinf(w * th) * dt;
+        for (int c = 0; c < 5; c++) {
+            const float p0 = c >= 3 ? ch0 : ce0, p1 = c >= 3 ? ch1 : ce1;
+            out[kf * 10 + 2 * c] += f[c] * p0;
+            out[kf * 10 + 2 * c + 1] += f[c] * p1;
+        }
+    }
+}
+
+// One NTFF patch gi at step n (cpu.rs:363-415): e = {Eu(p), Eu(p+v), Ev(p), Ev(p+u)} and
+// h = {Hu at its four nodes, Hv at its four nodes}, in the patch table's order.
+__device__ __forceinline__ void ntff_acc(const ObsArgs &a, uint64_t gi, uint32_t n, uint32_t meta, const float e[4],
+                                         const float h[8]) {
+    const float dt = a.dt;
+    const float te = (float(n) + 1.0f) * dt, th = (float(n) + 0.5f) * dt;
+    const uint32_t ax = meta & 3u;
+    const float side = (meta & 4u) ? 1.f : -1.f;
+    const uint32_t u = (ax + 1) % 3, v = (ax + 2) % 3;
+    float ef[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
+    ef[u] = 0.5f * (e[0] + e[1]);
+    ef[v] = 0.5f * (e[2] + e[3]);
+    hf[u] = 0.25f * (h[0] + h[1] + h[2] + h[3]);
+    hf[v] = 0.25f * (h[4] + h[5] + h[6] + h[7]);
+    float nrm[3] = {0.f, 0.f, 0.f};
+    nrm[ax] = side;
+    const float jv[3] = {nrm[1] * hf[2] - nrm[2] * hf[1], nrm[2] * hf[0] - nrm[0] * hf[2],
+                         nrm[0] * hf[1] - nrm[1] * hf[0]};
+    const float mv[3] = {ef[1] * nrm[2] - ef[2] * nrm[1], ef[2] * nrm[0] - ef[0] * nrm[2],
+                         ef[0] * nrm[1] - ef[1] * nrm[0]};
+    float *out = a.ntff + gi * a.nf * 12;
+    for (uint32_t kf = 0; kf < a.nf; kf++) {
+        const float wf = a.omega[kf];
+        // cos and sin are exactly even and odd, so the table's values are these
+        const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
+        const bool tab = n < a.n_phase;
+        const float pe0 = tab ? ph[0] : cosf(-wf * te) * dt, pe1 = tab ? ph[1] : sinf(-wf * te) * dt;
+        const float ph0 = tab ? ph[2] : cosf(-wf * th) * dt, ph1 = tab ? ph[3] : sinf(-wf * th) * dt;
+        float *o = out + kf * 12;
+        for (int c = 0; c < 3; c++) {
+            o[2 * c] += ph0 * jv[c];
+            o[2 * c + 1] += ph1 * jv[c];
+            o[6 + 2 * c] += pe0 * mv[c];
+            o[6 + 2 * c + 1] += pe1 * mv[c];
+        }
+    }
+}
+
 // Block `blk` of one run's observers (cpu.rs:309-415): port V/I waves, then point probes,
 // then plane DFT, then NTFF patches, each section a range of 256-thread blocks.
 __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const float *__restrict__ E,
@@ -701,27 +758,13 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
         return;
     }
     blk -= a.blk_probes;
-    const float dt = a.dt;
-    const float te = (float(n) + 1.0f) * dt, th = (float(n) + 0.5f) * dt;
     if (blk < a.blk_plane) {
         const uint32_t q = blk * 256 + threadIdx.x;
         if (q >= a.plane_n) return;
         const uint32_t i = q / g.n1, j = q % g.n1;
         const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + a.plane_k;
         const float f[5] = {E[id], E[nn + id], E[2 * nn + id], H[id], H[nn + id]};
-        float *out = a.plane + uint64_t(q) * a.nf * 10;
-        for (uint32_t kf = 0; kf < a.nf; kf++) {
-            const float w = a.omega[kf];
-            const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
-            const bool tab = n < a.n_phase;
-            const float ce0 = tab ? ph[0] : cosf(w * te) * dt, ce1 = tab ? ph[1] : -sinf(w * te) * dt;
-            const float ch0 = tab ? ph[2] : cosf(w * th) * dt, ch1 = tab ? ph[3] : -sinf(w * th) * dt;
-            for (int c = 0; c < 5; c++) {
-                const float p0 = c >= 3 ? ch0 : ce0, p1 = c >= 3 ? ch1 : ce1;
-                out[kf * 10 + 2 * c] += f[c] * p0;
-                out[kf * 10 + 2 * c + 1] += f[c] * p1;
-            }
-        }
+        plane_acc(a, q, n, f);
         return;
     }
     blk -= a.blk_plane;
@@ -729,37 +772,11 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
         const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
         if (gi >= a.n_patches) return;
         const uint32_t *ix = a.patch + gi * 13;
-        const uint32_t meta = ix[0];
-        const uint32_t ax = meta & 3u;
-        const float side = (meta & 4u) ? 1.f : -1.f;
-        const uint32_t u = (ax + 1) % 3, v = (ax + 2) % 3;
-        float ef[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
-        ef[u] = 0.5f * (E[u * nn + ix[1]] + E[u * nn + ix[2]]);
-        ef[v] = 0.5f * (E[v * nn + ix[3]] + E[v * nn + ix[4]]);
-        hf[u] = 0.25f * (H[u * nn + ix[5]] + H[u * nn + ix[6]] + H[u * nn + ix[7]] + H[u * nn + ix[8]]);
-        hf[v] = 0.25f * (H[v * nn + ix[9]] + H[v * nn + ix[10]] + H[v * nn + ix[11]] + H[v * nn + ix[12]]);
-        float nrm[3] = {0.f, 0.f, 0.f};
-        nrm[ax] = side;
-        const float jv[3] = {nrm[1] * hf[2] - nrm[2] * hf[1], nrm[2] * hf[0] - nrm[0] * hf[2],
-                             nrm[0] * hf[1] - nrm[1] * hf[0]};
-        const float mv[3] = {ef[1] * nrm[2] - ef[2] * nrm[1], ef[2] * nrm[0] - ef[0] * nrm[2],
-                             ef[0] * nrm[1] - ef[1] * nrm[0]};
-        float *out = a.ntff + gi * a.nf * 12;
-        for (uint32_t kf = 0; kf < a.nf; kf++) {
-            const float wf = a.omega[kf];
-            // cos and sin are exactly even and odd, so the table's values are these
-            const float *ph = a.phase + (uint64_t(n) * a.nf + kf) * 4;
-            const bool tab = n < a.n_phase;
-            const float pe0 = tab ? ph[0] : cosf(-wf * te) * dt, pe1 = tab ? ph[1] : sinf(-wf * te) * dt;
-            const float ph0 = tab ? ph[2] : cosf(-wf * th) * dt, ph1 = tab ? ph[3] : sinf(-wf * th) * dt;
-            float *o = out + kf * 12;
-            for (int c = 0; c < 3; c++) {
-                o[2 * c] += ph0 * jv[c];
-                o[2 * c + 1] += ph1 * jv[c];
-                o[6 + 2 * c] += pe0 * mv[c];
-                o[6 + 2 * c + 1] += pe1 * mv[c];
-            }
-        }
+        const uint32_t meta = ix[0], ax = meta & 3u, u = (ax + 1) % 3, v = (ax + 2) % 3;
+        const float e[4] = {E[u * nn + ix[1]], E[u * nn + ix[2]], E[v * nn + ix[3]], E[v * nn + ix[4]]};
+        const float h[8] = {H[u * nn + ix[5]], H[u * nn + ix[6]], H[u * nn + ix[7]], H[u * nn + ix[8]],
+                            H[v * nn + ix[9]], H[v * nn + ix[10]], H[v * nn + ix[11]], H[v * nn + ix[12]]};
+        ntff_acc(a, gi, n, meta, e, h);
     }
 }
 
@@ -797,11 +814,28 @@ struct Deb2 {
     float *eo;  // step n's final E per Debye edge
 };
 
-template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool DEB>
+// Step n's values at the points its plane DFT and NTFF observers read, captured inside the pass
+// by each point's owner into buf. Per (i, j) row, seg[start[r] .. start[r+1]) are segments of
+// consecutive k with the same fields: (k_lo, k_hi, mask of fields Ex Ey Ez Hx Hy Hz, first
+// index); field c at k sits at first + (k - k_lo) * popcount(mask) + (rank of c in mask). The
+// observers find their values through per-point index tables (ntff: 12 per patch, plane: 5 per
+// point), in the order the one-step observers read them.
+struct Cap2 {
+    const uint32_t *start;
+    const uint4 *seg;
+    float *buf;
+    const uint32_t *ntff;
+    const uint32_t *plane;
+};
+
+// SIDE: the first step has side work (Debye ADE through db, captures through cp; either may be
+// absent, NULL tables)
+template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool SIDE>
 __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
                                        float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
                                        const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
-                                       const Src2 &sr, float pl, const FixArgs &fx, const Deb2 &db) {
+                                       const Src2 &sr, float pl, const FixArgs &fx, const Deb2 &db,
+                                       const Cap2 &cp) {
     constexpr int ROWS = TJ + 4;
     constexpr bool PZ = PML >= 1, PXY = PML == 2;
     // row exchanges, double-buffered by plane parity; row w is stored at index w+1 so that
@@ -1010,9 +1044,9 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
         }
         // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
         // owner or halo), in the order of cpu.rs:284-297
-        const int drow = DEB && db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
+        const int drow = SIDE && db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
         int32_t dt3[3] = {-1, -1, -1};
-        if (DEB && drow >= 0) {
+        if (SIDE && drow >= 0) {
             const int32_t *dm = db.map + (uint64_t(uint32_t(drow)) * g.n2 + uk) * 3;
             for (int c = 0; c < 3; c++) dt3[c] = kin ? dm[c] : -1;
             for (int c = 0; c < 3; c++) {
@@ -1044,9 +1078,24 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
                 }
             }
         }
-        if (DEB && drow >= 0 && own && t >= i0 && t < i1)
+        if (SIDE && drow >= 0 && own && t >= i0 && t < i1)
             for (int c = 0; c < 3; c++)
                 if (dt3[c] >= 0) db.eo[dt3[c]] = c == 0 ? nex : c == 1 ? ney : nez;
+        // step n's final values at the observed points of this row, by their owner
+        if (SIDE && cp.start && t < g.n0 && jin) {
+            const uint64_t r = uint64_t(t) * g.n1 + uj;
+            const uint32_t q0 = cp.start[r], q1 = cp.start[r + 1];  // wave-uniform
+            const bool mine = own && t >= i0 && t < i1;
+            for (uint32_t q = q0; q < q1; q++) {
+                const uint4 sg = cp.seg[q];
+                if (mine && uk >= sg.x && uk < sg.y) {
+                    uint32_t at = sg.w + (uk - sg.x) * uint32_t(__popc(sg.z));
+                    const float v[6] = {nex, ney, nez, nhx, nhy, nhz};
+                    for (int c = 0; c < 6; c++)
+                        if ((sg.z >> c) & 1u) cp.buf[at++] = v[c];
+                }
+            }
+        }
         // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)
         const bool two = t >= 1 && t - 1 + 1 >= (i0 > 0 ? i0 : 1);
         const uint32_t s = t - 1;  // plane of step n+1
@@ -1239,7 +1288,7 @@ __device__ inline float field_mid(const Geo &g, const FixArgs &f, float pl, cons
 // field_mid above), block `blk` of the observer launch as in observe().
 __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs &f, const float *__restrict__ E,
                                    const float *__restrict__ H, const float *__restrict__ psi, uint32_t s_off,
-                                   uint32_t blk) {
+                                   uint32_t blk, const Cap2 &cp) {
     const uint32_t n = a.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
     const uint64_t nn = g.nn;
@@ -1268,6 +1317,27 @@ __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs
     if (blk < a.blk_probes) {
         const uint32_t q = blk * 256 + threadIdx.x;
         if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = field_mid(g, f, pl, E, H, psi, a.probe_f[q]);
+        return;
+    }
+    // plane DFT and NTFF from the values the pass captured (Cap2)
+    blk -= a.blk_probes;
+    if (blk < a.blk_plane) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q >= a.plane_n) return;
+        const uint32_t *ix = cp.plane + uint64_t(q) * 5;
+        const float fv[5] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]], cp.buf[ix[4]]};
+        plane_acc(a, q, n, fv);
+        return;
+    }
+    blk -= a.blk_plane;
+    if (blk < a.blk_ntff) {
+        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        if (gi >= a.n_patches) return;
+        const uint32_t *ix = cp.ntff + gi * 12;
+        const float e[4] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]]};
+        const float h[8] = {cp.buf[ix[4]], cp.buf[ix[5]], cp.buf[ix[6]], cp.buf[ix[7]],
+                            cp.buf[ix[8]], cp.buf[ix[9]], cp.buf[ix[10]], cp.buf[ix[11]]};
+        ntff_acc(a, gi, n, a.patch[gi * 13], e, h);
     }
 }
 

@@ -170,20 +170,21 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
                                                           const float *__restrict__ Eo, float *__restrict__ En,
                                                           const float *__restrict__ Ho, float *__restrict__ Hn,
                                                           const float *__restrict__ psio, float *__restrict__ psin,
-                                                          uint32_t seg, Src2 sr, FixArgs f, Deb2 db, uint32_t s_off) {
+                                                          uint32_t seg, Src2 sr, FixArgs f, Deb2 db, Cap2 cp,
+                                                          uint32_t s_off) {
     const MarchBlock mb = blocks[blockIdx.x];
     if (mb.bx == ~0u) return;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    march2<TJ, PML, WIDE, LDS_SETS, DEB>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db);
+    march2<TJ, PML, WIDE, LDS_SETS, DEB>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
 // The first step of a two-step launch exists only inside it; its observers recompute what they
 // read from the untouched previous copy (observe_mid).
 __global__ __launch_bounds__(256) void k_observe_mid(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ E,
                                                      const float *__restrict__ H, const float *__restrict__ psi,
-                                                     uint32_t s_off) {
-    observe_mid(g, a, f, E, H, psi, s_off, blockIdx.x);
+                                                     uint32_t s_off, Cap2 cp) {
+    observe_mid(g, a, f, E, H, psi, s_off, blockIdx.x, cp);
 }
 
 // ---- energy: sum E^2 + (376.73 H)^2 (cpu.rs:417-421, fdtd.wgsl:547-568) -----------------
@@ -357,6 +358,78 @@ static void plan_march(Session &s) {
     s.blk_int.upload(cls[0].data(), cls[0].size(), s.stream);
     s.blk_z.upload(cls[1].data(), cls[1].size(), s.stream);
     s.blk_pml.upload(cls[2].data(), cls[2].size(), s.stream);
+}
+
+// The points the first step of a two-step pass must hand to its plane-DFT and NTFF observers
+// (Cap2): every (field, node) they read, grouped per (i, j) row into segments of consecutive k
+// with the same fields, and the per-observer index tables into the capture buffer.
+static void capture_points(Session &s, const fg_problem *P, const fg_extras *X) {
+    hipStream_t st = s.stream;
+    const uint64_t nn = s.nn, n1 = P->n[1], n2 = P->n[2];
+    const ObsArgs &o = s.obs;
+    std::vector<uint64_t> pts;  // field * nn + id, field 0..5 = Ex Ey Ez Hx Hy Hz
+    std::vector<uint64_t> pkeys, nkeys;
+    if (o.plane_n)
+        for (uint32_t q = 0; q < o.plane_n; q++) {
+            const uint64_t id = (uint64_t(q / n1) * n1 + q % n1) * n2 + uint64_t(X->plane_k);
+            for (uint64_t c : {0, 1, 2, 3, 4}) pkeys.push_back(c * nn + id);
+        }
+    for (uint64_t g = 0; g < o.n_patches; g++) {
+        const uint32_t *ix = X->ntff + g * 13;
+        const uint64_t ax = ix[0] & 3u, u = (ax + 1) % 3, v = (ax + 2) % 3;
+        const uint64_t f[12] = {u, u, v, v, 3 + u, 3 + u, 3 + u, 3 + u, 3 + v, 3 + v, 3 + v, 3 + v};
+        for (int t = 0; t < 12; t++) nkeys.push_back(f[t] * nn + ix[1 + t]);
+    }
+    pts.insert(pts.end(), pkeys.begin(), pkeys.end());
+    pts.insert(pts.end(), nkeys.begin(), nkeys.end());
+    // per node id: the mask of captured fields
+    std::vector<std::pair<uint64_t, uint32_t>> nodes;  // (id, mask)
+    nodes.reserve(pts.size());
+    for (uint64_t key : pts) nodes.push_back({key % nn, 1u << uint32_t(key / nn)});
+    std::sort(nodes.begin(), nodes.end());
+    std::vector<std::pair<uint64_t, uint32_t>> merged;
+    for (const auto &p : nodes) {
+        if (!merged.empty() && merged.back().first == p.first) merged.back().second |= p.second;
+        else merged.push_back(p);
+    }
+    // segments: consecutive k of one row with one mask; node -> first index of its fields
+    std::vector<uint32_t> start(uint64_t(P->n[0]) * n1 + 1, 0);
+    std::vector<uint4> seg;
+    std::vector<uint64_t> first(merged.size());
+    uint64_t count = 0;
+    for (size_t q = 0; q < merged.size(); q++) {
+        const uint64_t id = merged[q].first, row = id / n2, k = id % n2;
+        const uint32_t mask = merged[q].second;
+        const bool extend = q > 0 && !seg.empty() && merged[q - 1].first + 1 == id && merged[q - 1].second == mask &&
+                            merged[q - 1].first / n2 == row;
+        if (extend) {
+            seg.back().y = uint32_t(k + 1);
+        } else {
+            seg.push_back(make_uint4(uint32_t(k), uint32_t(k + 1), mask, uint32_t(count)));
+            start[row + 1]++;
+        }
+        first[q] = count;
+        count += uint32_t(__builtin_popcount(mask));
+    }
+    if (count >= (1ull << 32)) throw HipError{"too many observed points for the two-step march", FG_TOO_LARGE};
+    for (size_t r = 1; r < start.size(); r++) start[r] += start[r - 1];
+    auto index = [&](uint64_t key) {
+        const uint64_t id = key % nn, c = key / nn;
+        const size_t q = size_t(std::lower_bound(merged.begin(), merged.end(), std::make_pair(id, 0u)) - merged.begin());
+        const uint32_t mask = merged[q].second;
+        return uint32_t(first[q] + uint32_t(__builtin_popcount(mask & ((1u << c) - 1u))));
+    };
+    std::vector<uint32_t> pidx(pkeys.size()), nidx(nkeys.size());
+    for (size_t q = 0; q < pkeys.size(); q++) pidx[q] = index(pkeys[q]);
+    for (size_t q = 0; q < nkeys.size(); q++) nidx[q] = index(nkeys[q]);
+    s.cap_start.upload(start.data(), start.size(), st);
+    s.cap_seg.upload(seg.data(), seg.size(), st);
+    s.cap_buf.alloc(std::max<uint64_t>(count, 1));
+    if (pidx.empty()) s.cap_plane.alloc(1);
+    else s.cap_plane.upload(pidx.data(), pidx.size(), st);
+    if (nidx.empty()) s.cap_ntff.alloc(1);
+    else s.cap_ntff.upload(nidx.data(), nidx.size(), st);
+    FG_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
 }
 
 Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse,
@@ -641,16 +714,26 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         }
     }
     const char *f2 = getenv("FIELDGRID_FUSE2");
-    s.fuse2 = s.fused && !(f2 && f2[0] == '0') && !observed_debye && f.n_ind == 0 && !f.serial && o.plane_n == 0 &&
-              o.n_patches == 0 && P->n_sheets == 0 && P->n_wires == 0;
+    s.fuse2 = s.fused && !(f2 && f2[0] == '0') && !observed_debye && f.n_ind == 0 && !f.serial && P->n_sheets == 0 &&
+              P->n_wires == 0;
     if (s.fuse2) {
         s.seg2 = 32;
         if (const char *v = getenv("FIELDGRID_SEG2")) s.seg2 = std::max(1u, uint32_t(atoi(v)));
         s.seg2 = std::min<uint32_t>(s.seg2, SEG_MAX);
-        if (!getenv("FIELDGRID_SEG2"))
-            while (s.seg2 > 4 && uint64_t((P->n[2] + OWN_K2 - 1) / OWN_K2) * ((P->n[1] + s.tj2 - 1) / s.tj2) *
-                                     ((P->n[0] + s.seg2 - 1) / s.seg2) < 3ull * uint64_t(std::max(dev->cus, 1)))
+        // Each block class is its own launch of 1024-thread blocks, one or two per CU: halve the
+        // segment while the largest class would give the GPU fewer than two rounds of blocks, or
+        // the whole pass fewer than three blocks per CU (a short last round idles most CUs).
+        if (!getenv("FIELDGRID_SEG2")) {
+            const uint64_t cus = uint64_t(std::max(dev->cus, 1));
+            std::vector<MarchBlock> cls[3];
+            while (s.seg2 > 4) {
+                march_classes2(s, s.tj2, s.seg2, cls);
+                const uint64_t big = std::max({cls[0].size(), cls[1].size(), cls[2].size()}),
+                               total = cls[0].size() + cls[1].size() + cls[2].size();
+                if (total >= 3 * cus && big >= 2 * cus) break;
                 s.seg2 /= 2;
+            }
+        }
         // the driven port's sources by plane (i), for the first step inside the launch
         std::vector<std::vector<uint4>> by_plane(P->n[0]);
         if (P->n_ports)
@@ -691,6 +774,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
             s.deb2_jn.alloc(std::max<uint64_t>(uint64_t(P->n_poles) * P->n_debye, 1));
             s.deb2_eo.alloc(P->n_debye);
         }
+        if (o.plane_n || o.n_patches) capture_points(s, P, X);
         plan_march2(s);
     }
     FG_CHECK(hipStreamSynchronize(st));
@@ -852,19 +936,20 @@ void launch_pair(Session &s, uint32_t s_off) {
     const MarchBlock *bi = s.blk2_int.p, *bz = s.blk2_z.p, *bp = s.blk2_pml.p;
     const Src2 sr{s.src2_start.p, s.src2_ent.p};
     const Deb2 db{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
+    const Cap2 cp{s.cap_start.p, s.cap_seg.p, s.cap_buf.p, s.cap_ntff.p, s.cap_plane.p};
 #define FG_MARCH2X(TJ, W, L, D)                                                                                      \
     do {                                                                                                             \
-        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
-        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L, D>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
-        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L, D>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, s_off); \
+        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
+        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L, D>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
+        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L, D>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
     } while (0)
 #define FG_MARCH2D(W, L)                                     \
     do {                                                    \
-        if (db.row) FG_MARCH2X(12, W, L, true);             \
+        if (db.row || cp.start) FG_MARCH2X(12, W, L, true); \
         else FG_MARCH2X(12, W, L, false);                   \
     } while (0)
     {
-        // runs without Debye media take the kernel without the in-pass ADE (fewer registers)
+        // runs without Debye media or captured observers take the leaner kernel
         const bool lds = g.n_sets <= uint32_t(SETS_LDS);
         if (g.wide) {
             if (lds) FG_MARCH2D(true, true); else FG_MARCH2D(true, false);
@@ -882,7 +967,7 @@ void launch_pair(Session &s, uint32_t s_off) {
     if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, f2, En, Eo, s_off + 1);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 2), st));
     if (s.obs_blocks && s.observe) {
-        hipLaunchKernelGGL(k_observe_mid, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, s_off);
+        hipLaunchKernelGGL(k_observe_mid, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, s_off, cp);
         hipLaunchKernelGGL(k_observe, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, En, Hn, s_off + 1);
     }
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 3), st));

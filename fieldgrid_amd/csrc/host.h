@@ -111,6 +111,9 @@ struct Session {
     DevBuf<uint4> src2_ent;
     DevBuf<int32_t> deb2_row, deb2_map;  // Debye k maps of the two-step march (Deb2)
     DevBuf<float> deb2_jn, deb2_eo;
+    DevBuf<uint32_t> cap_start, cap_ntff, cap_plane;  // observed points of the two-step march (Cap2)
+    DevBuf<uint4> cap_seg;
+    DevBuf<float> cap_buf;
     // resistive sheets (records restored from sheets0 on reset) and thin wires
     SheetArgs sheet{};
     DevBuf<float> sheets, sheet_cur, branches;

@@ -289,6 +289,27 @@ def test_two_steps_per_launch_with_debye_media_change_nothing(gpu, monkeypatch, 
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seg2", ["32", "3"])
+def test_two_steps_per_launch_with_ntff_and_plane_dft_change_nothing(gpu, monkeypatch, seg2):
+    """The first step's plane-DFT and NTFF observers read values the pass captured at their points
+    (Cap2: each point's owner writes step n's E/H): the boxes recipe at 56^3 with Debye boxes,
+    NTFF at three frequencies and a plane DFT, against one step per pass and the FP32 oracle,
+    bit for bit."""
+    sim = W.boxes(nodes=56, pml=6, nboxes=8, ndebye=4, per_box_tau=True)
+    pulse = W.boxes_pulse()
+    ex = W.boxes_extras(sim, steps=351, inset=2, check_every=100, plane_k=23, probes=[(2, 30, 28, 20), (3, 25, 30, 31)])
+    monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    monkeypatch.delenv("FIELDGRID_FUSE2", raising=False)
+    two = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    monkeypatch.setenv("FIELDGRID_FUSE2", "0")
+    one = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    for name in ("series", "probes", "plane", "ntff", "e", "h"):
+        a, b = getattr(two, name), getattr(one, name)
+        assert a.size > 0 and np.array_equal(a, b), f"{name}: {int(np.sum(a != b))} of {a.size} differ"
+    _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"boxes with NTFF and plane DFT, seg2 {seg2}")
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("seg2", ["32", "5", "1"])
 def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2):
     """march2 (two steps per pass, the intermediate step in registers) against one step per
