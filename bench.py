@@ -282,6 +282,7 @@ def main():
     sess.step(args.timed_steps)
     ms, timed = sess.timing(False)
     energy = sess.energy()
+    two = sess.steps_per_pass() == 2  # temporal blocking (march2) for this problem
     sess.close()
     if not (energy == energy) or energy == float("inf"):
         print("the FDTD run went unstable", file=sys.stderr)
@@ -291,8 +292,6 @@ def main():
     # algorithmic bytes of the dominant (sweep) kernel per launch: E/H/coef/psi traffic only
     core_bpn = W.bytes_per_node_step(sim, None, core=True)
     achieved = core_bpn * nn / (sweep_ms * 1e-3) / 1e9
-    # two steps per launch (march2) when nothing but port sources acts between steps
-    two = args.engine == "fused" and os.environ.get("FIELDGRID_FUSE2", "1") != "0" and args.workload in ("layered", "plain")
     # HBM traffic per step of the sweep from the committed rocprofv3 PMC summary of this workload
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"k_march_{args.workload}_traffic.json")

@@ -1127,6 +1127,14 @@ int fg_session_timing(fg_session *ss, int enable, double ms[3], uint64_t *steps)
     });
 }
 
+int fg_session_steps_per_pass(const fg_session *ss, uint32_t *out) {
+    return fg::guarded([&] {
+        if (!ss || !out) throw fg::HipError{"fg_session_steps_per_pass: NULL argument", FG_INVALID};
+        *out = reinterpret_cast<const fg::Session *>(ss)->fuse2 ? 2u : 1u;
+        return int(FG_OK);
+    });
+}
+
 int fg_session_sync(fg_session *ss) {
     return fg::guarded([&] {
         FG_CHECK(hipStreamSynchronize(reinterpret_cast<fg::Session *>(ss)->stream));

@@ -768,6 +768,12 @@ class Session:
     def sync(self):
         abi.check(abi.lib().fg_session_sync(self._h))
 
+    def steps_per_pass(self) -> int:
+        """Steps one pass of the sweep advances (2 with temporal blocking, else 1)."""
+        v = C.c_uint32()
+        abi.check(abi.lib().fg_session_steps_per_pass(self._h, C.byref(v)))
+        return int(v.value)
+
     def timing(self, enable: bool):
         """Switch per-kernel HIP-event timing; returns ([sweep, fix, observe] ms, steps) so far."""
         ms = (C.c_double * 3)()

@@ -226,6 +226,11 @@ int fg_session_step(fg_session *s, uint64_t count);
    [step sweep(s), sparse fixes, observers] and the number of steps timed, then switches
    timing on (enable != 0) or off and restarts the totals. */
 int fg_session_timing(fg_session *s, int enable, double ms[3], uint64_t *steps);
+/* Steps the session's sweep advances per pass over HBM: 2 where only port sources and Debye
+   media act between steps and no port or probe reads a Debye edge (temporal blocking), else 1.
+   No reference counterpart (the reference runs one step per dispatch chain); for callers that
+   account bytes or time per pass. */
+int fg_session_steps_per_pass(const fg_session *s, uint32_t *out);
 /* Wait for every step enqueued so far. */
 int fg_session_sync(fg_session *s);
 /* Sum of E^2 + (376.73 H)^2 over the grid, synchronously (run.rs:716-735). */
