@@ -837,16 +837,17 @@ static void forward(fg_adjoint &A, float *series) {
 // the forward march: one halo row/lane each side and one warm-up plane).
 static void adj_march(Session &s, const float *Lo, float *Ln, const float *Mo, float *Mn, const float *po, float *pn,
                       const GradArgs &ga, const InjectArgs &in) {
-    hipStream_t st = s.stream;
     const Geo &g = s.geo;
     const dim3 block(64, s.tj + 2);
     const uint32_t ni = uint32_t(s.blk_int.n), nz = uint32_t(s.blk_z.n), np_ = uint32_t(s.blk_pml.n);
     const MarchBlock *bi = s.blk_int.p, *bz = s.blk_z.p, *bp = s.blk_pml.p;
+    hipStream_t cs[3];
+    fork_classes(s, cs);
 #define FG_AM2(TJ, W, L)                                                                                                     \
     do {                                                                                                                     \
-        if (ni) hipLaunchKernelGGL((k_adj_march<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
-        if (nz) hipLaunchKernelGGL((k_adj_march<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
-        if (np_) hipLaunchKernelGGL((k_adj_march<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+        if (ni) hipLaunchKernelGGL((k_adj_march<TJ, 0, W, L>), dim3(ni), block, 0, cs[0], g, bi, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+        if (nz) hipLaunchKernelGGL((k_adj_march<TJ, 1, W, L>), dim3(nz), block, 0, cs[1], g, bz, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
+        if (np_) hipLaunchKernelGGL((k_adj_march<TJ, 2, W, L>), dim3(np_), block, 0, cs[2], g, bp, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
     } while (0)
 #define FG_AM(TJ)                                                                     \
     do {                                                                              \
@@ -861,6 +862,7 @@ static void adj_march(Session &s, const float *Lo, float *Ln, const float *Mo, f
     else FG_AM(14);
 #undef FG_AM
 #undef FG_AM2
+    join_classes(s);
 }
 
 static void backward(fg_adjoint &A, const double *dseries, double *grad) {

@@ -461,6 +461,17 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     }
     FG_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     hipStream_t st = s.stream;
+    // concurrent block classes: on a 16.7 M-node grid the classes' last rounds overlap for ~10 %;
+    // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves
+    const char *cc = getenv("FIELDGRID_STREAMS");
+    const bool side = cc ? cc[0] != '0' : nn >= (8ull << 20);
+    if (s.fused && side) {
+        for (int q = 0; q < 2; q++) {
+            FG_CHECK(hipStreamCreateWithFlags(&s.side[q], hipStreamNonBlocking));
+            FG_CHECK(hipEventCreateWithFlags(&s.join[q], hipEventDisableTiming));
+        }
+        FG_CHECK(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
+    }
     for (int c = 0; c < copies; c++) {
         s.E[c].alloc(3 * nn);
         s.H[c].alloc(3 * nn);
@@ -829,6 +840,24 @@ static void collect_timing(Session &s) {
     s.units.clear();
 }
 
+void fork_classes(Session &s, hipStream_t out[3]) {
+    out[0] = out[1] = out[2] = s.stream;
+    if (!s.side[0]) return;
+    FG_CHECK(hipEventRecord(s.fork, s.stream));
+    for (int q = 0; q < 2; q++) {
+        FG_CHECK(hipStreamWaitEvent(s.side[q], s.fork, 0));
+        out[1 + q] = s.side[q];
+    }
+}
+
+void join_classes(Session &s) {
+    if (!s.side[0]) return;
+    for (int q = 0; q < 2; q++) {
+        FG_CHECK(hipEventRecord(s.join[q], s.side[q]));
+        FG_CHECK(hipStreamWaitEvent(s.stream, s.join[q], 0));
+    }
+}
+
 // Enqueue one step at offset s_off from the device step counter.
 void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
     hipStream_t st = s.stream;
@@ -846,11 +875,13 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
         float *En = s.E[b].p, *Hn = s.H[b].p, *Pn = s.psi[b].p;
         const uint32_t ni = uint32_t(s.blk_int.n), nz = uint32_t(s.blk_z.n), np_ = uint32_t(s.blk_pml.n);
         const MarchBlock *bi = s.blk_int.p, *bz = s.blk_z.p, *bp = s.blk_pml.p;
+        hipStream_t cs[3];
+        fork_classes(s, cs);
 #define FG_MARCH2(TJ, W, L)                                                                                       \
     do {                                                                                                          \
-        if (ni) hipLaunchKernelGGL((k_march<TJ, 0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg); \
-        if (nz) hipLaunchKernelGGL((k_march<TJ, 1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg); \
-        if (np_) hipLaunchKernelGGL((k_march<TJ, 2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (ni) hipLaunchKernelGGL((k_march<TJ, 0, W, L>), dim3(ni), block, 0, cs[0], g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (nz) hipLaunchKernelGGL((k_march<TJ, 1, W, L>), dim3(nz), block, 0, cs[1], g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg); \
+        if (np_) hipLaunchKernelGGL((k_march<TJ, 2, W, L>), dim3(np_), block, 0, cs[2], g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg); \
     } while (0)
 #define FG_MARCH(TJ)                                                   \
     do {                                                               \
@@ -865,9 +896,9 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
             const WireArgs wa{s.wire_start.p, s.wire_ent.p, s.wires.p};
 #define FG_MARCHW(W, L)                                                                                            \
     do {                                                                                                           \
-        if (ni) hipLaunchKernelGGL((k_march_wires<0, W, L>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
-        if (nz) hipLaunchKernelGGL((k_march_wires<1, W, L>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
-        if (np_) hipLaunchKernelGGL((k_march_wires<2, W, L>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+        if (ni) hipLaunchKernelGGL((k_march_wires<0, W, L>), dim3(ni), block, 0, cs[0], g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+        if (nz) hipLaunchKernelGGL((k_march_wires<1, W, L>), dim3(nz), block, 0, cs[1], g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
+        if (np_) hipLaunchKernelGGL((k_march_wires<2, W, L>), dim3(np_), block, 0, cs[2], g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg, wa); \
     } while (0)
             const bool lds = g.n_sets <= uint32_t(SETS_LDS);
             if (g.wide) {
@@ -884,6 +915,7 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step) {
         }
 #undef FG_MARCH2
 #undef FG_MARCH
+        join_classes(s);
         if (s.sheet.n)
             hipLaunchKernelGGL(k_sheet, dim3(s.sheet.serial ? 1u : uint32_t((s.sheet.n + 255) / 256)), dim3(256), 0, st,
                                s.sheet, En, Eo, Hn);
@@ -937,11 +969,13 @@ void launch_pair(Session &s, uint32_t s_off) {
     const Src2 sr{s.src2_start.p, s.src2_ent.p};
     const Deb2 db{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
     const Cap2 cp{s.cap_start.p, s.cap_seg.p, s.cap_buf.p, s.cap_ntff.p, s.cap_plane.p};
+    hipStream_t cs[3];
+    fork_classes(s, cs);
 #define FG_MARCH2X(TJ, W, L, D)                                                                                      \
     do {                                                                                                             \
-        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, st, g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
-        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L, D>), dim3(nz), block, 0, st, g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
-        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L, D>), dim3(np_), block, 0, st, g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
+        if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, cs[0], g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
+        if (nz) hipLaunchKernelGGL((k_march2<TJ, 1, W, L, D>), dim3(nz), block, 0, cs[1], g, bz, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
+        if (np_) hipLaunchKernelGGL((k_march2<TJ, 2, W, L, D>), dim3(np_), block, 0, cs[2], g, bp, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
     } while (0)
 #define FG_MARCH2D(W, L)                                     \
     do {                                                    \
@@ -959,6 +993,7 @@ void launch_pair(Session &s, uint32_t s_off) {
     }
 #undef FG_MARCH2D
 #undef FG_MARCH2X
+    join_classes(s);
     s.cur = b;
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
     // the second step's sparse fixes: Debye from the pole states and step-n E the pass wrote

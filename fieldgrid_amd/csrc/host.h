@@ -129,8 +129,18 @@ struct Session {
     std::vector<int> slots_host;  // per axis: slot_e row, slot_h row, nmax entries each
     std::vector<float> ind_i0;    // initial inductor currents
 
+    // the block classes of one pass run concurrently on stream, side[0] and side[1] (forked and
+    // joined with events), so each class's last round of blocks overlaps the others
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+
     ~Session() {
         for (auto e : events) (void)hipEventDestroy(e);
+        for (int q = 0; q < 2; q++) {
+            if (side[q]) (void)hipStreamSynchronize(side[q]), (void)hipStreamDestroy(side[q]);
+            if (join[q]) (void)hipEventDestroy(join[q]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -181,6 +191,11 @@ void enqueue_steps(Session &s, uint64_t count);
 // step at offset s_off from it (the caller advances s.steps afterwards).
 void set_step_counter(Session &s);
 void launch_step(Session &s, uint32_t s_off, uint64_t global_step);
+// Streams for the three block classes of a pass (interior, z-slab, x/y-slab): forked from
+// the session stream when the session has side streams, else the session stream three times;
+// join_classes makes the session stream wait for them.
+void fork_classes(Session &s, hipStream_t out[3]);
+void join_classes(Session &s);
 // Two steps (s_off, s_off + 1) in one march2 launch; only when s.fuse2.
 void launch_pair(Session &s, uint32_t s_off);
 
