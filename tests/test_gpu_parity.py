@@ -93,9 +93,9 @@ def test_plain_vacuum_fields_mid_run(gpu, engine):
 @pytest.mark.parametrize("driven", [0, 1])
 @pytest.mark.parametrize("engine", ENGINES, ids=lambda e: NAMES[e])
 def test_every_device_feature_matches_the_oracle(gpu, engine, driven):
-    """tests/fdtd.rs:147-174 without sheets and wires (which the engine refuses): Debye band
-    poles, PEC, C/R/L lumped, two multi-column ports, plane DFT, NTFF; both driven ports."""
-    sim = everything(sheets=False, wires=False)
+    """tests/fdtd.rs:147-174: Debye band poles, PEC, C/R/L lumped, two multi-column ports,
+    resistive sheets, thin wires, plane DFT, NTFF; both driven ports."""
+    sim = everything()
     ex = everything_extras(sim, 1500)
     ex.engine = engine
     pulse = F.Pulse(5e9, 5e9)
