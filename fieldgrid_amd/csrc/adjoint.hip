@@ -737,12 +737,14 @@ struct fg_adjoint {
     fg::DevBuf<float> snaps, lamE, lamA, lamH, lamH2, lamP[2];
     fg::DevBuf<uint64_t> d_edges, d_obs;
     fg::DevBuf<int32_t> dmap;
-    fg::DevBuf<float2> tape;
+    fg::DevBuf<float2> tape[2];  // design-edge tapes of two segments (recompute k-1 while the adjoint reads k)
     fg::DevBuf<double> gca, gcb, dseries;
     uint64_t n_snaps = 0, snap_len = 0;
     double ms[4] = {0, 0, 0, 0};
+    hipStream_t astream = nullptr;  // the adjoint sweeps' stream (the session stream recomputes)
 
     ~fg_adjoint() {
+        if (astream) (void)hipStreamSynchronize(astream), (void)hipStreamDestroy(astream);
         s.reset();
         if (sim) fg_sim_free(sim);
     }
@@ -835,14 +837,13 @@ static void forward(fg_adjoint &A, float *series) {
 
 // One adjoint step through k_adj_march over the session's march block lists (the same reach as
 // the forward march: one halo row/lane each side and one warm-up plane).
-static void adj_march(Session &s, const float *Lo, float *Ln, const float *Mo, float *Mn, const float *po, float *pn,
-                      const GradArgs &ga, const InjectArgs &in) {
+static void adj_march(Session &s, hipStream_t st, const float *Lo, float *Ln, const float *Mo, float *Mn,
+                      const float *po, float *pn, const GradArgs &ga, const InjectArgs &in) {
     const Geo &g = s.geo;
     const dim3 block(64, s.tj + 2);
     const uint32_t ni = uint32_t(s.blk_int.n), nz = uint32_t(s.blk_z.n), np_ = uint32_t(s.blk_pml.n);
     const MarchBlock *bi = s.blk_int.p, *bz = s.blk_z.p, *bp = s.blk_pml.p;
-    hipStream_t cs[3];
-    fork_classes(s, cs);
+    const hipStream_t cs[3] = {st, st, st};
 #define FG_AM2(TJ, W, L)                                                                                                     \
     do {                                                                                                                     \
         if (ni) hipLaunchKernelGGL((k_adj_march<TJ, 0, W, L>), dim3(ni), block, 0, cs[0], g, bi, Lo, Ln, Mo, Mn, po, pn, s.seg, ga, in); \
@@ -862,7 +863,6 @@ static void adj_march(Session &s, const float *Lo, float *Ln, const float *Mo, f
     else FG_AM(14);
 #undef FG_AM
 #undef FG_AM2
-    join_classes(s);
 }
 
 static void backward(fg_adjoint &A, const double *dseries, double *grad) {
@@ -884,7 +884,8 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
             o[q] = A.obs[q].field * nn + (uint64_t(A.obs[q].i) * s.n[1] + A.obs[q].j) * s.n[2] + A.obs[q].k;
         A.d_edges.upload(f.data(), f.size(), st);
         A.d_obs.upload(o.data(), o.size(), st);
-        A.tape.alloc(std::max<uint64_t>(A.every * ne, 1));
+        A.tape[0].alloc(std::max<uint64_t>(A.every * ne, 1));
+        A.tape[1].alloc(std::max<uint64_t>(A.every * ne, 1));
         A.gca.alloc(std::max<uint64_t>(ne, 1));
         A.gcb.alloc(std::max<uint64_t>(ne, 1));
         // design edge per (component, node of the box's nodes lo..hi)
@@ -909,8 +910,18 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
     A.dseries.upload(dseries, A.steps * n_obs, st);
-    std::vector<hipEvent_t> ev(3 * A.n_snaps);
+    // The session stream recomputes the segments (restore, forward with tape); the adjoint
+    // stream sweeps them backward. Segment k-1 is recomputed while the adjoint sweeps segment k:
+    // the tapes alternate between two buffers, and the recompute of segment k waits until the
+    // adjoint of segment k+2 has read the buffer it overwrites. FIELDGRID_ADJ_OVERLAP=0 runs
+    // both on the session stream.
+    const char *ov = getenv("FIELDGRID_ADJ_OVERLAP");
+    if (!A.astream) FG_CHECK(hipStreamCreateWithFlags(&A.astream, hipStreamNonBlocking));
+    hipStream_t as = ov && ov[0] == '0' ? st : A.astream;
+    // per segment: recompute start/end on st, adjoint start/end on as; plus the setup event
+    std::vector<hipEvent_t> ev(4 * A.n_snaps + 1);
     for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
+    hipEvent_t ready = ev[4 * A.n_snaps];
     const dim3 block(64, 4), grid((g.n2 + 63) / 64, (g.n1 + 3) / 4, g.n0);
     const uint32_t eb = uint32_t((ne + 255) / 256);
     GradArgs ga{};
@@ -919,13 +930,17 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     ga.map = A.dmap.p, ga.gca = A.gca.p, ga.gcb = A.gcb.p;
     int pin = 0;
     const auto t0 = std::chrono::steady_clock::now();
+    FG_CHECK(hipEventRecord(ready, st));
+    FG_CHECK(hipStreamWaitEvent(as, ready, 0));
     // dJ/ds of the last step; every earlier one enters at the end of the next step's transpose
     if (n_obs && A.steps)
-        hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, st, g, A.lamE.p, march ? nullptr : A.lamA.p, A.d_obs.p,
+        hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, as, g, A.lamE.p, march ? nullptr : A.lamA.p, A.d_obs.p,
                            uint32_t(n_obs), A.dseries.p + (A.steps - 1) * n_obs);
     for (uint64_t k = A.n_snaps; k-- > 0;) {
         const uint64_t start = k * A.every, end = std::min(start + A.every, A.steps);
-        FG_CHECK(hipEventRecord(ev[3 * k], st));
+        float2 *tape = A.tape[k & 1].p;
+        if (k + 2 < A.n_snaps) FG_CHECK(hipStreamWaitEvent(st, ev[4 * (k + 2) + 3], 0));
+        FG_CHECK(hipEventRecord(ev[4 * k], st));
         // restore the segment's starting fields into copy 0
         if (k == 0) {
             s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
@@ -946,42 +961,45 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
                 if (ne)
                     hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
                                        s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
-                                       A.tape.p + (n - start) * ne, A.tape.p + (n + 1 - start) * ne);
+                                       tape + (n - start) * ne, tape + (n + 1 - start) * ne);
                 n += 2;
             } else {
                 launch_step(s, uint32_t(n - start), n);
                 if (ne)
                     hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
-                                       s.H[s.cur].p, A.tape.p + (n - start) * ne);
+                                       s.H[s.cur].p, tape + (n - start) * ne);
                 n += 1;
             }
         }
         s.steps = end;
         s.observe = true;
-        FG_CHECK(hipEventRecord(ev[3 * k + 1], st));
+        FG_CHECK(hipEventRecord(ev[4 * k + 1], st));
+        FG_CHECK(hipStreamWaitEvent(as, ev[4 * k + 1], 0));
+        FG_CHECK(hipEventRecord(ev[4 * k + 2], as));
         for (uint64_t n = end; n-- > start;) {
-            ga.tape = A.tape.p + (n - start) * ne;
+            ga.tape = tape + (n - start) * ne;
             in.ds = n ? A.dseries.p + (n - 1) * n_obs : nullptr;
             if (march) {
-                adj_march(s, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga, in);
+                adj_march(s, as, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga, in);
                 le ^= 1;
             } else {
-                hipLaunchKernelGGL(k_adj_e, grid, block, 0, st, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
-                hipLaunchKernelGGL(k_adj_h, grid, block, 0, st, g, A.lamE.p, A.lamA.p, A.lamH.p, A.lamP[pin].p,
+                hipLaunchKernelGGL(k_adj_e, grid, block, 0, as, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
+                hipLaunchKernelGGL(k_adj_h, grid, block, 0, as, g, A.lamE.p, A.lamA.p, A.lamH.p, A.lamP[pin].p,
                                    A.lamP[pin ^ 1].p, in);
             }
             pin ^= 1;
         }
-        FG_CHECK(hipEventRecord(ev[3 * k + 2], st));
+        FG_CHECK(hipEventRecord(ev[4 * k + 3], as));
         FG_CHECK(hipGetLastError());
     }
+    FG_CHECK(hipStreamSynchronize(as));
     FG_CHECK(hipStreamSynchronize(st));
     A.ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    A.ms[2] = A.ms[3] = 0.0;
+    A.ms[2] = A.ms[3] = 0.0;  // busy time of each stream (they overlap)
     for (uint64_t k = 0; k < A.n_snaps; k++) {
         float a = 0.f, b = 0.f;
-        FG_CHECK(hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]));
-        FG_CHECK(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
+        FG_CHECK(hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]));
+        FG_CHECK(hipEventElapsedTime(&b, ev[4 * k + 2], ev[4 * k + 3]));
         A.ms[2] += a, A.ms[3] += b;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

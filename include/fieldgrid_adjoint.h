@@ -13,8 +13,9 @@
  * in the fields, so the adjoint recursion runs backward in time on its own fields. The forward
  * fields enter only through E and the curl term on the design edges. The forward pass keeps a
  * snapshot of the fields every `every` steps. The backward pass recomputes one segment at a time
- * from its snapshot, taping the design-edge values, and then runs the adjoint over that segment.
- * Memory is the snapshots plus one segment's tape.
+ * from its snapshot, taping the design-edge values, and runs the adjoint over that segment on a
+ * second stream while the next (earlier) segment is recomputed. Memory is the snapshots plus two
+ * segments' tapes.
  *
  * Supported problems: curl updates, CPML and resistive ports. Debye media, lumped inductors,
  * resistive sheets and thin wires give FG_UNSUPPORTED. The box must not touch a port, lumped or
@@ -56,8 +57,10 @@ int fg_adjoint_forward(fg_adjoint *adj, const float *eps_box, float *series);
 int fg_adjoint_backward(fg_adjoint *adj, const double *dseries, double *grad);
 /* f64 forward run of the same problem with the box's eps_r = eps_box (NULL = current). */
 int fg_adjoint_forward64(fg_adjoint *adj, const double *eps_box, double *series);
-/* Wall times (ms) of the last forward and backward passes; the backward split into the
-   recompute of the segments and the adjoint steps (HIP events); snapshots kept. */
+/* Wall times (ms) of the last forward and backward passes, then the busy times (HIP events) of
+   the backward's two streams: the recompute of the segments and the adjoint steps. The backward
+   recomputes segment k-1 while the adjoint sweeps segment k, so the two overlap and their sum
+   exceeds the backward's wall time. Also the number of snapshots kept. */
 int fg_adjoint_timing(const fg_adjoint *adj, double ms[4], uint64_t *snapshots);
 void fg_adjoint_free(fg_adjoint *adj);
 
