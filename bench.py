@@ -156,7 +156,8 @@ def sweep(args):
                   "sequential_mcells": round(node_steps / seq_s / 1e6, 2),
                   "records_identical": same,
                   "tune": {"tj": tune.used[0], "seg": tune.used[1], "chunk": tune.used[2],
-                           "tune_ms": round(tune.tune_ms, 1), "from_cache": tune.from_cache}},
+                           "tune_ms": round(tune.tune_ms, 1), "kept_steps": tune.kept_steps,
+                           "from_cache": tune.from_cache}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None, "alg_bytes_per_launch": round(alg),

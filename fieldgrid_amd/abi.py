@@ -124,7 +124,7 @@ class fg_sim_desc(C.Structure):
 class fg_tune(C.Structure):
     _fields_ = [("tj", u32), ("seg", u32), ("chunk", u32), ("depth", u32), ("cache", C.c_char_p),
                 ("used_tj", u32), ("used_seg", u32), ("used_chunk", u32), ("from_cache", u32),
-                ("used_depth", u32), ("pad", u32), ("tune_ms", f64), ("step_ms", f64)]
+                ("used_depth", u32), ("kept_steps", u32), ("tune_ms", f64), ("step_ms", f64)]
 
 
 class fg_spectra(C.Structure):

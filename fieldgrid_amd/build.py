@@ -20,10 +20,10 @@ ARCH = os.environ.get("FIELDGRID_ARCH", "gfx950")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function",
+    "-Wall", "-Wno-unused-function", "-Werror=return-type",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall",
-              "-Wno-missing-field-initializers"]
+              "-Wno-missing-field-initializers", "-Werror=return-type"]
 
 
 def _run(cmd):

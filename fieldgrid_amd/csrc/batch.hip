@@ -362,11 +362,14 @@ static hipEvent_t event(Batch &b, size_t i) {
     return b.ev[i];
 }
 
-// Mean ms per step over `steps` steps after `warm` untimed ones (graphs as in a real run).
-static double time_steps(Batch &b, uint32_t warm, uint32_t steps, bool graphs) {
-    advance(b, 0, warm, graphs);
+// Mean ms per step over `steps` steps after `warm` untimed ones (graphs as in a real run),
+// from step `at`, which moves on by warm + steps.
+static double time_steps(Batch &b, uint64_t &at, uint32_t warm, uint32_t steps, bool graphs) {
+    advance(b, at, warm, graphs);
+    at += warm;
     FG_CHECK(hipEventRecord(event(b, 0), b.stream));
-    advance(b, warm, steps, graphs);
+    advance(b, at, steps, graphs);
+    at += steps;
     FG_CHECK(hipEventRecord(event(b, 1), b.stream));
     FG_CHECK(hipEventSynchronize(event(b, 1)));
     float ms = 0.f;
@@ -432,7 +435,13 @@ static std::string shape_key(const Batch &b) {
     return k.str();
 }
 
-static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
+// Pick the tile, depth and graph chunk (from the cache, or by timing candidates: at most 174
+// steps, a 4-step probe, up to 9 candidates of 2 + 8 steps and the graph-chunk trials). The
+// trial steps are real steps of every run, since every tiling gives bit-identical fields: when
+// they end before first_check (the earliest decay check or last step over the runs), the runs
+// keep them and the returned count is the step the batch goes on from; otherwise every run is
+// reset and 0 is returned.
+static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uint64_t first_check) {
     const auto t0 = std::chrono::steady_clock::now();
     const std::string key = shape_key(b);
     auto cache = read_cache(t ? t->cache : nullptr);
@@ -446,6 +455,7 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
     b.chunk = t && t->chunk ? t->chunk : 32;
     bool from_cache = false, tuned = false;
     double step_ms = 0.0;
+    uint64_t at = 0;  // steps the trial passes have taken
     auto it = cache.find(key);
     const bool ask = ask_tj || ask_seg || ask_chunk || ask_depth;
     if (ask && it != cache.end()) {
@@ -472,15 +482,18 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
             for (uint32_t tj : {14u, 8u}) {
                 if (depth == 2 && tj != 14) continue;  // the two-step march has one tile height
                 if (depth == 1 && !ask_tj && tj != b.tj) continue;
+                // where two steps per pass are possible, one step per pass is only tried with
+                // 14-row tiles (8-row tiles re-read more halo and have not won there)
+                if (depth == 1 && tj == 8 && ask_depth && ask_tj) continue;
                 for (uint32_t seg : {32u, 16u, 8u}) {
                     if (!ask_seg && seg != seg_in) continue;
                     b.depth = depth, b.tj = tj, b.seg = b.seg_base = b.seg2 = b.seg2_base = seg;
                     plan(b, active);
                     if (!steps) {
-                        const double one = time_steps(b, 2, 2, false);
+                        const double one = time_steps(b, at, 2, 2, false);
                         steps = 2 * std::min<uint32_t>(4, std::max<uint32_t>(1, uint32_t(20.0 / one)));
                     }
-                    const double ms = time_steps(b, 2, steps, false);
+                    const double ms = time_steps(b, at, 2, steps, false);
                     if (ms < best) best = ms, btj = tj, bseg = seg, bdepth = depth;
                 }
             }
@@ -496,18 +509,21 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
             if (best < 1.0) {
                 b.chunk = 8;
                 b.drop_graph();
-                const double c8 = time_steps(b, 8, 8, true);
+                const double c8 = time_steps(b, at, 8, 8, true);
                 b.chunk = 32;
                 b.drop_graph();
-                const double c32 = time_steps(b, 32, 32, true);
+                const double c32 = time_steps(b, at, 32, 32, true);
                 b.chunk = c32 < 0.98 * c8 ? 32 : 8;
                 step_ms = std::min(step_ms, std::min(c8, c32));
             }
             b.drop_graph();
         }
         FG_CHECK(hipStreamSynchronize(b.stream));
-        for (auto &s : b.runs) reset(*s);
-        b.par = 0;
+        if (at >= first_check) {
+            for (auto &s : b.runs) reset(*s);
+            b.par = 0;
+            at = 0;
+        }
         cache[key] = {double(b.tj), double(b.depth == 2 ? b.seg2_base : b.seg_base), double(b.chunk), step_ms,
                       double(b.depth)};
         write_cache(t ? t->cache : nullptr, cache);
@@ -522,8 +538,10 @@ static void autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active) {
         t->used_tj = b.depth == 2 ? TJ2 : b.tj, t->used_seg = b.depth == 2 ? b.seg2_base : b.seg_base;
         t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0, t->used_depth = b.depth;
         t->step_ms = step_ms;
+        t->kept_steps = uint32_t(at);
         t->tune_ms = tuned ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
     }
+    return at;
 }
 
 static void spectra(Batch &b, uint32_t count, fg_spectra *sp) {
@@ -603,7 +621,6 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         b.table.upload(table.data(), table.size(), b.stream);
         std::vector<uint8_t> active(count, 0);
         for (uint32_t r = 0; r < count; r++) active[r] = extras[r]->max_steps > 0;
-        if (count) autotune(b, tune, active);
         // per-run decay checks at multiples of check_every and at max_steps (run.rs:681-748)
         std::vector<uint64_t> every(count), pulse_end(count);
         for (uint32_t r = 0; r < count; r++) {
@@ -614,13 +631,18 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         auto next_check = [&](uint32_t r, uint64_t at) {
             return std::min((at / every[r] + 1) * every[r], extras[r]->max_steps);
         };
+        uint64_t first_check = UINT64_MAX;
+        for (uint32_t r = 0; r < count; r++)
+            if (active[r]) first_check = std::min(first_check, next_check(r, 0));
+        // the batch goes on from the tuner's trial steps when it kept them
+        uint64_t global = count ? autotune(b, tune, active, first_check) : 0;
         const bool graphs = b.chunk >= 2;
-        uint64_t global = 0;
         bool dirty = false;
         while (std::find(active.begin(), active.end(), 1) != active.end()) {
             uint64_t next = UINT64_MAX;
             for (uint32_t r = 0; r < count; r++)
                 if (active[r]) next = std::min(next, next_check(r, global));
+            if (next <= global) throw HipError{"fg_run_batch: no step left before the next check", FG_INVALID};
             if (dirty) plan(b, active), dirty = false;
             advance(b, global, next - global, graphs);
             global = next;

@@ -568,6 +568,7 @@ class Tune:
     from_cache: bool = False
     tune_ms: float = 0.0
     step_ms: float = 0.0
+    kept_steps: int = 0  # trial steps the runs kept (the batch went on from there)
 
 
 def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
@@ -631,6 +632,7 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
         tune.used = (t.used_tj, t.used_seg, t.used_chunk, t.used_depth)
         tune.from_cache = bool(t.from_cache)
         tune.tune_ms, tune.step_ms = t.tune_ms, t.step_ms
+        tune.kept_steps = int(t.kept_steps)
     records = [_finish(recs[r], bufs_all[r], runs[r].sim, runs[r].extras) for r in range(count)]
     spectra = []
     for r, sb in enumerate(spec_bufs):

@@ -37,8 +37,12 @@ typedef struct fg_tune {
                              updated after tuning; NULL = no cache                              */
     /* out */
     uint32_t used_tj, used_seg, used_chunk, from_cache;
-    uint32_t used_depth, pad;
-    double tune_ms;       /* time spent tuning (0 when the cache or the caller decided)         */
+    uint32_t used_depth;
+    uint32_t kept_steps;  /* steps the tuner's timed trials advanced every run by and the batch
+                             went on from (0: the trials were rolled back, because a run would
+                             have reached a decay check or its last step within them)          */
+    double tune_ms;       /* time spent tuning (0 when the cache or the caller decided),
+                             including the kept trial steps                                    */
     double step_ms;       /* mean time of one step of the whole batch, as the tuner measured it */
 } fg_tune;
 

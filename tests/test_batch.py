@@ -106,6 +106,8 @@ def test_batched_runs_equal_single_runs(gpu, tune):
     print(f"  tuned: tj, seg, chunk, depth = {tune.used}, tune {tune.tune_ms:.1f} ms, step {tune.step_ms:.3f} ms")
     if tune.depth:
         assert tune.used[3] == tune.depth
+    if not (tune.tj or tune.seg or tune.chunk or tune.depth):
+        assert tune.kept_steps == 0  # a decay check at step 40 falls inside the trials: rolled back
     for q, r in enumerate(runs):
         single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
         _same(recs[q], single, f"run {q}")
@@ -113,6 +115,21 @@ def test_batched_runs_equal_single_runs(gpu, tune):
         V2, I2 = O.port_spectra(single.series, len(r.sim.ports), r.sim.dt, r.freqs)
         assert np.max(np.abs(V - V2)) <= 1e-9 * max(np.max(np.abs(V2)), 1e-300)
         assert np.max(np.abs(I - I2)) <= 1e-9 * max(np.max(np.abs(I2)), 1e-300)
+
+
+@pytest.mark.gpu
+def test_the_tuners_trial_steps_are_kept_as_progress(gpu):
+    """With no decay check or last step inside the tuner's trials, the runs keep the trial steps
+    (one and two steps per pass, direct and graph-replayed, switching between candidates) and
+    the batch goes on from there: every record still bit-identical to a single run."""
+    runs = small_sweep(count=4, steps=(400, 520))
+    tune = F.Tune()
+    recs, _ = F.run_batch(gpu, runs, tune=tune, fields=True)
+    print(f"\ntuned {tune.used}, kept {tune.kept_steps} trial steps, tune {tune.tune_ms:.1f} ms")
+    assert not tune.from_cache and 0 < tune.kept_steps < 400
+    for q, r in enumerate(runs):
+        single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
+        _same(recs[q], single, f"run {q}")
 
 
 @pytest.mark.gpu
