@@ -904,6 +904,51 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
     if (w < 2)
         for (int x = 0; x < 4; x++)
             for (int q = 0; q < 2; q++) xr[x][q][w * (ROWS + 1)][0][lane] = 0.f, xr[x][q][w * (ROWS + 1)][1][lane] = 0.f;
+    // The per-plane entries of the source, Debye-row and capture tables, staged in LDS before the
+    // march: the plane loop then reads them from LDS. Read from global memory inside the loop, a
+    // table entry's wait (vmcnt counts every vector memory access in issue order) would also
+    // wait for the plane loads issued one iteration ahead, once per plane.
+    //   xsrc[t - tstart]: sr.start[t] for the planes tstart .. i1 + 1 (clamped to n0);
+    //   xdeb[(t - tstart) * ROWS + w]: the Debye row slot of (t, row w), -1 outside the grid;
+    //   xcap[(t - i0) * (TJ + 1) + r]: cp.start of owned row r of owned plane t (r = TJ: the end),
+    //   xseg: the owned planes' capture segments, plane p's at xcoff[p] on, when they all fit.
+    constexpr uint32_t CAP_SEGS = SIDE ? 1536 : 1;
+    __shared__ uint32_t xsrc[SEG_MAX + 4];
+    __shared__ int32_t xdeb[SIDE ? (SEG_MAX + 3) * ROWS : 1];
+    __shared__ uint32_t xcap[SIDE ? SEG_MAX * (TJ + 1) : 1];
+    __shared__ uint32_t xcoff[SIDE ? SEG_MAX + 1 : 1];
+    __shared__ uint4 xseg[CAP_SEGS];
+    for (uint32_t t = w * 64 + lane; t + tstart <= i1 + 1; t += 64 * ROWS) xsrc[t] = sr.start[min(tstart + t, g.n0)];
+    bool seg_lds = false;
+    if constexpr (SIDE) {
+        const uint32_t np = i1 - tstart + 1, no = i1 - i0, j0 = mb.by * TJ;
+        for (uint32_t x = w * 64 + lane; x < np * ROWS; x += 64 * ROWS) {
+            const uint32_t t = tstart + x / ROWS;
+            const int jj = int(j0) + int(x % ROWS) - 2;
+            xdeb[x] = db.row && t < g.n0 && jj >= 0 && jj < int(g.n1) ? db.row[uint64_t(t) * g.n1 + uint32_t(jj)] : -1;
+        }
+        if (cp.start)
+            for (uint32_t x = w * 64 + lane; x < no * (TJ + 1); x += 64 * ROWS)
+                xcap[x] = cp.start[uint64_t(i0 + x / (TJ + 1)) * g.n1 + min(j0 + x % (TJ + 1), g.n1)];
+        __syncthreads();
+        if (cp.start && w == 0 && lane == 0) {
+            uint32_t o = 0;
+            for (uint32_t q = 0; q < no; q++) xcoff[q] = o, o += xcap[q * (TJ + 1) + TJ] - xcap[q * (TJ + 1)];
+            xcoff[no] = o;
+        }
+        __syncthreads();
+        seg_lds = cp.start && xcoff[no] <= CAP_SEGS;  // block-uniform
+        if (seg_lds)
+            for (uint32_t x = w * 64 + lane; x < xcoff[no]; x += 64 * ROWS) {
+                uint32_t lo = 0, hi = no;  // the owned plane of entry x: xcoff[lo] <= x < xcoff[hi]
+                while (hi - lo > 1) {
+                    const uint32_t m = (lo + hi) / 2;
+                    if (xcoff[m] <= x) lo = m;
+                    else hi = m;
+                }
+                xseg[x] = cp.seg[xcap[lo * (TJ + 1)] + (x - xcoff[lo])];
+            }
+    }
     // plane p of a field array through a resource that is empty when p is outside the grid
     auto pbase = [&](const float *a, int64_t p) { return a + (p >= 0 && p < int64_t(g.n0) ? uint64_t(p) * plane : 0); };
     auto pbytes = [&](int64_t p, uint32_t bytes) { return p >= 0 && p < int64_t(g.n0) ? bytes : 0u; };
@@ -1044,7 +1089,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
         }
         // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
         // owner or halo), in the order of cpu.rs:284-297
-        const int drow = SIDE && db.row && t < g.n0 && jin ? db.row[uint64_t(t) * g.n1 + uj] : -1;  // wave-uniform
+        const int drow = SIDE ? xdeb[(t - tstart) * ROWS + w] : -1;  // wave-uniform
         int32_t dt3[3] = {-1, -1, -1};
         if (SIDE && drow >= 0) {
             const int32_t *dm = db.map + (uint64_t(uint32_t(drow)) * g.n2 + uk) * 3;
@@ -1067,7 +1112,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
             }
         }
         if (t < g.n0) {
-            const uint32_t s0 = sr.start[t], s1 = sr.start[t + 1];
+            const uint32_t s0 = xsrc[t - tstart], s1 = xsrc[t - tstart + 1];
             for (uint32_t q = s0; q < s1; q++) {
                 const uint4 e = sr.ent[q];
                 if (int(e.x) == j && int(e.y) == k) {
@@ -1082,12 +1127,12 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
             for (int c = 0; c < 3; c++)
                 if (dt3[c] >= 0) db.eo[dt3[c]] = c == 0 ? nex : c == 1 ? ney : nez;
         // step n's final values at the observed points of this row, by their owner
-        if (SIDE && cp.start && t < g.n0 && jin) {
-            const uint64_t r = uint64_t(t) * g.n1 + uj;
-            const uint32_t q0 = cp.start[r], q1 = cp.start[r + 1];  // wave-uniform
-            const bool mine = own && t >= i0 && t < i1;
+        if (SIDE && cp.start && t >= i0 && t < i1 && w >= 2 && w < 2 + TJ && jin) {  // wave-uniform
+            const uint32_t *hp = xcap + (t - i0) * (TJ + 1);
+            const uint32_t q0 = hp[w - 2], q1 = hp[w - 1], at0 = xcoff[t - i0] - hp[0];
+            const bool mine = own;
             for (uint32_t q = q0; q < q1; q++) {
-                const uint4 sg = cp.seg[q];
+                const uint4 sg = seg_lds ? xseg[q + at0] : cp.seg[q];
                 if (mine && uk >= sg.x && uk < sg.y) {
                     uint32_t at = sg.w + (uk - sg.x) * uint32_t(__popc(sg.z));
                     const float v[6] = {nex, ney, nez, nhx, nhy, nhz};
