@@ -448,7 +448,7 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
             oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
         }
         // lamH: + terms at the node, - terms at +z (next lane), +y (row below, LDS), +x (next plane)
-        const float x1z = __shfl_down(cur.hv1, 1, 64), x0z = __shfl_down(cur.hu0, 1, 64);
+        const float x1z = lane_next(cur.hv1), x0z = lane_next(cur.hu0);
         const float x0y = xv[b][w + 2][0][lane], x2y = xv[b][w + 2][1][lane];
         const float mx = hx0 + ((cur.hv1 - x1z) + (cur.hu2 - x2y));
         const float my = hy0 + ((cur.hv2 - nxt.hv2) + (cur.hu0 - x0z));
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
         xv[b ^ 1][w + 1][0][lane] = nxt.hv0, xv[b ^ 1][w + 1][1][lane] = nxt.hu2;
         __syncthreads();
         // lamE: ca lamE + terms at -z (previous lane), -y (row above, LDS), -x (plane before)
-        const float y1z = __shfl_up(ev1, 1, 64), y0z = __shfl_up(eu0, 1, 64);
+        const float y1z = lane_prev(ev1), y0z = lane_prev(eu0);
         const float y0y = xw[b][w][0][lane], y2y = xw[b][w][1][lane];
         float le0 = cur.f0.y != 0.f ? cur.lx * cur.f0.x : cur.lx;
         float le1 = cur.f1.y != 0.f ? cur.ly * cur.f1.x : cur.ly;

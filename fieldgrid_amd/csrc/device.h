@@ -150,6 +150,17 @@ struct MarchBlock {
     uint32_t bx, by, bz;
 };
 
+// The +k / -k neighbour's value across the wave's 64 lanes as one DPP move (wave_shl:1 /
+// wave_shr:1, a VALU op instead of an LDS-routed ds_bpermute): lane l gets lane l+1's (l-1's)
+// value; lane 63 (lane 0), which has no such neighbour, keeps its own, as __shfl_down(v, 1) and
+// __shfl_up(v, 1) return. Called with all 64 lanes active.
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
 // Raw buffer access: a plane-uniform base pointer in a buffer resource (scalar registers)
 // and a 32-bit per-lane byte offset, so no lane carries a 64-bit address. `bytes` bounds the
 // resource: an access past it reads 0 / is dropped by the hardware instead of faulting.
@@ -370,7 +381,7 @@ __device__ __forceinline__ void march(const Geo &g, const MarchBlock mb, const f
         // new psi from its old value; stored (where the slot exists) only by the node's owner
 #define CP(base, bytes, o, B, CC, R, old, diff) cpml_next(psin + (base), bytes, sto == OOB ? OOB : (o), B, CC, R, old, diff)
         // k+1 neighbours of E(i) from the next lane
-        const float ex_k = __shfl_down(ex, 1, 64), ey_k = __shfl_down(ey, 1, 64);
+        const float ex_k = lane_next(ex), ey_k = lane_next(ey);
         float hx, hy, hz;
         {
             // Hx: u = y, v = z
@@ -425,7 +436,7 @@ __device__ __forceinline__ void march(const Geo &g, const MarchBlock mb, const f
         const float hx_j = xh[b][w][0][lane], hz_j = xh[b][w][1][lane];
         const float exn1 = xe[b][w + 2][0][lane], ezn1 = xe[b][w + 2][1][lane];
         // k-1 neighbours of new H(i) from the previous lane
-        const float hx_k = __shfl_up(hx, 1, 64), hy_k = __shfl_up(hy, 1, 64);
+        const float hx_k = lane_prev(hx), hy_k = lane_prev(hy);
         {
             float *q = En + pb;
             // Ex: u = y, v = z
@@ -1034,7 +1045,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
         }
         // ---- step n, H at plane t (the +1 neighbours of E0: j+1 from LDS, k+1 from the next lane)
         const float ex_j = xr[0][b ^ 1][w + 2][0][lane], ez_j = xr[0][b ^ 1][w + 2][1][lane];
-        const float ex_k = __shfl_down(ex0, 1, 64), ey_k = __shfl_down(ey0, 1, 64);
+        const float ex_k = lane_next(ex0), ey_k = lane_next(ey0);
         float nhx, nhy, nhz;
         {
             const float ev = ez_j - ez0, eu = ey_k - ey0;
@@ -1063,7 +1074,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
         // ---- step n, E at plane t (the -1 neighbours of step-n H: j-1 from LDS, k-1 from the
         // previous lane, i-1 from the previous iteration)
         const float hx_j = xr[1][b][w][0][lane], hz_j = xr[1][b][w][1][lane];
-        const float hx_k = __shfl_up(nhx, 1, 64), hy_k = __shfl_up(nhy, 1, 64);
+        const float hx_k = lane_prev(nhx), hy_k = lane_prev(nhy);
         const float2 f0 = coef(c0), f1 = coef(c1), f2 = coef(c2);
         float nex, ney, nez;
         {
@@ -1164,7 +1175,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
             const bool wr = own && s >= i0;  // this thread stores plane s
             const uint32_t sto = wr ? stO : OOB;
             const float ex_j2 = xr[2][b ^ 1][w + 2][0][lane], ez_j2 = xr[2][b ^ 1][w + 2][1][lane];
-            const float ex_k2 = __shfl_down(e1x, 1, 64), ey_k2 = __shfl_down(e1y, 1, 64);
+            const float ex_k2 = lane_next(e1x), ey_k2 = lane_next(e1y);
             float ohx, ohy, ohz;
             {
                 const float ev = ez_j2 - e1z, eu = ey_k2 - e1y;
@@ -1214,7 +1225,7 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
             xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
             __syncthreads();
             const float hx_j2 = xr[3][b][w][0][lane], hz_j2 = xr[3][b][w][1][lane];
-            const float hx_k2 = __shfl_up(ohx, 1, 64), hy_k2 = __shfl_up(ohy, 1, 64);
+            const float hx_k2 = lane_prev(ohx), hy_k2 = lane_prev(ohy);
             float *q = En + uint64_t(s) * plane;
             {
                 const float hv = ohz - hz_j2, hu = ohy - hy_k2;
