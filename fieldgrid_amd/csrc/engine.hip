@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,61 @@
 namespace fg {
 
 thread_local std::string g_error;
+thread_local Stager *t_stager = nullptr;
+
+namespace {
+std::mutex g_pin_mutex;
+std::vector<char *> g_pin_free;  // pinned chunks of Stager::CHUNK bytes not in use
+}  // namespace
+
+hipStream_t take_stream(fg_device *dev) {
+    {
+        std::lock_guard<std::mutex> lock(dev->stream_mutex);
+        if (!dev->idle_streams.empty()) {
+            hipStream_t s = dev->idle_streams.back();
+            dev->idle_streams.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    FG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void give_stream(fg_device *dev, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(dev->stream_mutex);
+    dev->idle_streams.push_back(s);
+}
+
+Stager::Stager(hipStream_t s) : stream_(s), prev_(t_stager) { t_stager = this; }
+
+Stager::~Stager() {
+    t_stager = prev_;
+    if (chunks_.empty()) return;
+    (void)hipStreamSynchronize(stream_);  // the DMAs have read the chunks
+    std::lock_guard<std::mutex> lock(g_pin_mutex);
+    for (char *c : chunks_) g_pin_free.push_back(c);
+}
+
+bool Stager::copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (s != stream_ || bytes > MAX_BYTES) return false;
+    const size_t at = (used_ + 255) & ~size_t(255);
+    if (at + bytes > CHUNK) {
+        char *c = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(g_pin_mutex);
+            if (!g_pin_free.empty()) c = g_pin_free.back(), g_pin_free.pop_back();
+        }
+        if (!c) FG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c), CHUNK, hipHostMallocDefault));
+        chunks_.push_back(c);
+        used_ = 0;
+        return copy(dst, src, bytes, s);
+    }
+    std::memcpy(chunks_.back() + at, src, bytes);
+    FG_CHECK(hipMemcpyAsync(dst, chunks_.back() + at, bytes, hipMemcpyHostToDevice, s));
+    used_ = at + bytes;
+    return true;
+}
 
 int set_error(int status, const std::string &msg) {
     g_error = msg;
@@ -459,15 +515,16 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         const uint64_t cells = uint64_t(P->n[0] - 1) * (P->n[1] - 1) * (P->n[2] - 1);
         throw HipError{std::to_string(cells) + " cells is more than this GPU can hold", FG_TOO_LARGE};
     }
-    FG_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.stream = take_stream(dev);
     hipStream_t st = s.stream;
+    Stager stage(st);  // the tables below go up through pinned memory
     // concurrent block classes: on a 16.7 M-node grid the classes' last rounds overlap for ~10 %;
     // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves
     const char *cc = getenv("FIELDGRID_STREAMS");
     const bool side = cc ? cc[0] != '0' : nn >= (8ull << 20);
     if (s.fused && side) {
         for (int q = 0; q < 2; q++) {
-            FG_CHECK(hipStreamCreateWithFlags(&s.side[q], hipStreamNonBlocking));
+            s.side[q] = take_stream(dev);
             FG_CHECK(hipEventCreateWithFlags(&s.join[q], hipEventDisableTiming));
         }
         FG_CHECK(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
@@ -1104,7 +1161,11 @@ int fg_open(int ordinal, fg_device **out) {
     });
 }
 
-void fg_close(fg_device *dev) { delete dev; }
+void fg_close(fg_device *dev) {
+    if (!dev) return;
+    for (hipStream_t s : dev->idle_streams) (void)hipStreamDestroy(s);
+    delete dev;
+}
 
 const char *fg_name(const fg_device *dev) { return dev->name.c_str(); }
 

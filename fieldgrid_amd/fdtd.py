@@ -656,6 +656,7 @@ class Design:
                  pulse: Pulse, steps: int, every: int = 100):
         if gpu is None:
             raise ValueError("Design needs a Gpu")
+        self.gpu = gpu  # the device outlives the adjoint's sessions
         self.desc = SimDesc(grid, mats, pec, list(lumped), list(ports), list(resistive), media or Media())
         self.lo, self.hi = tuple(lo), tuple(hi)
         self.shape = tuple(h - l for l, h in zip(lo, hi))
@@ -830,6 +831,7 @@ class Far:
                  scale=(1.0, 0.0)):
         if gpu is None:
             raise ValueError("Far needs a Gpu")
+        self.gpu = gpu  # the device outlives this object's buffers
         pos = np.ascontiguousarray(p.pos, np.float64)
         area = np.ascontiguousarray(p.area, np.float64)
         acc = np.ascontiguousarray(acc, np.float32)

@@ -195,7 +195,8 @@ typedef struct fg_device fg_device;
 
 /* Gpu::open / open_blocking (gpu.rs:86-108): open HIP device `ordinal`. */
 int fg_open(int ordinal, fg_device **out);
-/* drop(Gpu) */
+/* drop(Gpu). Sessions, adjoint and far-field objects opened on `dev` are freed first: they
+   hand their streams back to the device, which destroys them here. */
 void fg_close(fg_device *dev);
 /* Gpu.name (gpu.rs:36-41) */
 const char *fg_name(const fg_device *dev);
