@@ -85,12 +85,13 @@ __global__ __launch_bounds__(256) void k_fix_batch(const BatchRun *__restrict__ 
              uint64_t(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
-// observers of the first step of a two-step pass, from the copy the pass read
-__global__ __launch_bounds__(256) void k_observe_mid_batch(const BatchRun *__restrict__ runs,
-                                                           const uint32_t *__restrict__ list, uint32_t par,
-                                                           uint32_t s_off) {
+// both steps' observers of a two-step pass (observe_pair; the batch's pairs capture nothing)
+__global__ __launch_bounds__(256) void k_observe_pair_batch(const BatchRun *__restrict__ runs,
+                                                            const uint32_t *__restrict__ list, uint32_t par,
+                                                            uint32_t s_off) {
     const BatchRun &r = runs[list[blockIdx.y]];
-    observe_mid(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], s_off, blockIdx.x, Cap2{});
+    observe_pair(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], r.E[par ^ 1u], r.H[par ^ 1u], s_off, blockIdx.x,
+                 Cap2{});
 }
 
 __global__ __launch_bounds__(256) void k_observe_batch(const BatchRun *__restrict__ runs,
@@ -294,12 +295,9 @@ static void launch_pair(Batch &b, uint32_t s_off, uint32_t par) {
     if (b.n_fix)
         hipLaunchKernelGGL(k_fix_batch, dim3(b.fix_gx, b.n_fix), dim3(256), 0, b.stream, b.table.p, b.fix_list.p, par,
                            s_off + 1, 1u);
-    if (b.n_obs) {
-        hipLaunchKernelGGL(k_observe_mid_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p,
+    if (b.n_obs)
+        hipLaunchKernelGGL(k_observe_pair_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p,
                            b.obs_list.p, par, s_off);
-        hipLaunchKernelGGL(k_observe_batch, dim3(b.obs_gx, b.n_obs), dim3(256), 0, b.stream, b.table.p, b.obs_list.p,
-                           par, s_off + 1);
-    }
 }
 
 // Capture `chunk` steps starting from copy 0, ending on copy 0 with the counter advanced by

@@ -1397,4 +1397,123 @@ __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs
     }
 }
 
+// Both steps' observers of a two-step pass in one launch: port V/I and probes as observe_mid
+// (step n, from the input copy) followed by observe (step n+1, from the output copy); the plane
+// DFT and NTFF accumulators read once and written once, with step n's term added before step
+// n+1's, the order the two separate observer launches add them in, so every accumulator is
+// bit-identical to theirs and its traffic is halved.
+__device__ __forceinline__ void plane_acc2(const ObsArgs &a, uint32_t q, uint32_t n, const float f0[5],
+                                           const float f1[5]) {
+    const float dt = a.dt;
+    float *out = a.plane + uint64_t(q) * a.nf * 10;
+    for (uint32_t kf = 0; kf < a.nf; kf++) {
+        const float w = a.omega[kf];
+        float ce[2][2], ch[2][2];
+        for (int s = 0; s < 2; s++) {
+            const uint32_t m = n + uint32_t(s);
+            const float te = (float(m) + 1.0f) * dt, th = (float(m) + 0.5f) * dt;
+            const float *ph = a.phase + (uint64_t(m) * a.nf + kf) * 4;
+            const bool tab = m < a.n_phase;
+            ce[s][0] = tab ? ph[0] : cosf(w * te) * dt, ce[s][1] = tab ? ph[1] : -sinf(w * te) * dt;
+            ch[s][0] = tab ? ph[2] : cosf(w * th) * dt, ch[s][1] = tab ? ph[3] : -sinf(w * th) * dt;
+        }
+        for (int c = 0; c < 5; c++)
+            for (int r = 0; r < 2; r++) {
+                float x = out[kf * 10 + 2 * c + r];
+                x += f0[c] * (c >= 3 ? ch[0][r] : ce[0][r]);
+                x += f1[c] * (c >= 3 ? ch[1][r] : ce[1][r]);
+                out[kf * 10 + 2 * c + r] = x;
+            }
+    }
+}
+
+// J = n x H and M = E x n of one patch (ntff_acc's terms)
+__device__ __forceinline__ void ntff_terms(uint32_t meta, const float e[4], const float h[8], float jv[3], float mv[3]) {
+    const uint32_t ax = meta & 3u;
+    const float side = (meta & 4u) ? 1.f : -1.f;
+    const uint32_t u = (ax + 1) % 3, v = (ax + 2) % 3;
+    float ef[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
+    ef[u] = 0.5f * (e[0] + e[1]);
+    ef[v] = 0.5f * (e[2] + e[3]);
+    hf[u] = 0.25f * (h[0] + h[1] + h[2] + h[3]);
+    hf[v] = 0.25f * (h[4] + h[5] + h[6] + h[7]);
+    float nrm[3] = {0.f, 0.f, 0.f};
+    nrm[ax] = side;
+    jv[0] = nrm[1] * hf[2] - nrm[2] * hf[1], jv[1] = nrm[2] * hf[0] - nrm[0] * hf[2], jv[2] = nrm[0] * hf[1] - nrm[1] * hf[0];
+    mv[0] = ef[1] * nrm[2] - ef[2] * nrm[1], mv[1] = ef[2] * nrm[0] - ef[0] * nrm[2], mv[2] = ef[0] * nrm[1] - ef[1] * nrm[0];
+}
+
+__device__ __forceinline__ void ntff_acc2(const ObsArgs &a, uint64_t gi, uint32_t n, uint32_t meta, const float e0[4],
+                                          const float h0[8], const float e1[4], const float h1[8]) {
+    const float dt = a.dt;
+    float jv[2][3], mv[2][3];
+    ntff_terms(meta, e0, h0, jv[0], mv[0]);
+    ntff_terms(meta, e1, h1, jv[1], mv[1]);
+    float *out = a.ntff + gi * a.nf * 12;
+    for (uint32_t kf = 0; kf < a.nf; kf++) {
+        const float wf = a.omega[kf];
+        float pe[2][2], ph[2][2];
+        for (int s = 0; s < 2; s++) {
+            const uint32_t m = n + uint32_t(s);
+            const float te = (float(m) + 1.0f) * dt, th = (float(m) + 0.5f) * dt;
+            const float *p = a.phase + (uint64_t(m) * a.nf + kf) * 4;
+            const bool tab = m < a.n_phase;
+            pe[s][0] = tab ? p[0] : cosf(-wf * te) * dt, pe[s][1] = tab ? p[1] : sinf(-wf * te) * dt;
+            ph[s][0] = tab ? p[2] : cosf(-wf * th) * dt, ph[s][1] = tab ? p[3] : sinf(-wf * th) * dt;
+        }
+        float *o = out + kf * 12;
+        for (int c = 0; c < 3; c++)
+            for (int r = 0; r < 2; r++) {
+                float x = o[2 * c + r];
+                x += ph[0][r] * jv[0][c];
+                x += ph[1][r] * jv[1][c];
+                o[2 * c + r] = x;
+                float y = o[6 + 2 * c + r];
+                y += pe[0][r] * mv[0][c];
+                y += pe[1][r] * mv[1][c];
+                o[6 + 2 * c + r] = y;
+            }
+    }
+}
+
+__device__ inline void observe_pair(const Geo &g, const ObsArgs &a, const FixArgs &f, const float *__restrict__ Eo,
+                                    const float *__restrict__ Ho, const float *__restrict__ psio,
+                                    const float *__restrict__ En, const float *__restrict__ Hn, uint32_t s_off,
+                                    uint32_t blk, const Cap2 &cp) {
+    if (blk < a.blk_ports + a.blk_probes) {
+        observe_mid(g, a, f, Eo, Ho, psio, s_off, blk, cp);
+        observe(g, a, En, Hn, s_off + 1, blk);
+        return;
+    }
+    const uint32_t n = a.step[0] + s_off;
+    const uint64_t nn = g.nn;
+    blk -= a.blk_ports + a.blk_probes;
+    if (blk < a.blk_plane) {
+        const uint32_t q = blk * 256 + threadIdx.x;
+        if (q >= a.plane_n) return;
+        const uint32_t *ix = cp.plane + uint64_t(q) * 5;
+        const float f0[5] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]], cp.buf[ix[4]]};
+        const uint32_t i = q / g.n1, j = q % g.n1;
+        const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + a.plane_k;
+        const float f1[5] = {En[id], En[nn + id], En[2 * nn + id], Hn[id], Hn[nn + id]};
+        plane_acc2(a, q, n, f0, f1);
+        return;
+    }
+    blk -= a.blk_plane;
+    if (blk < a.blk_ntff) {
+        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        if (gi >= a.n_patches) return;
+        const uint32_t *cx = cp.ntff + gi * 12;
+        const float e0[4] = {cp.buf[cx[0]], cp.buf[cx[1]], cp.buf[cx[2]], cp.buf[cx[3]]};
+        const float h0[8] = {cp.buf[cx[4]], cp.buf[cx[5]], cp.buf[cx[6]], cp.buf[cx[7]],
+                             cp.buf[cx[8]], cp.buf[cx[9]], cp.buf[cx[10]], cp.buf[cx[11]]};
+        const uint32_t *ix = a.patch + gi * 13;
+        const uint32_t meta = ix[0], ax = meta & 3u, u = (ax + 1) % 3, v = (ax + 2) % 3;
+        const float e1[4] = {En[u * nn + ix[1]], En[u * nn + ix[2]], En[v * nn + ix[3]], En[v * nn + ix[4]]};
+        const float h1[8] = {Hn[u * nn + ix[5]], Hn[u * nn + ix[6]], Hn[u * nn + ix[7]], Hn[u * nn + ix[8]],
+                             Hn[v * nn + ix[9]], Hn[v * nn + ix[10]], Hn[v * nn + ix[11]], Hn[v * nn + ix[12]]};
+        ntff_acc2(a, gi, n, meta, e0, h0, e1, h1);
+    }
+}
+
 }  // namespace fg

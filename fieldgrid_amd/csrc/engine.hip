@@ -235,12 +235,14 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
     march2<TJ, PML, WIDE, LDS_SETS, DEB>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
-// The first step of a two-step launch exists only inside it; its observers recompute what they
-// read from the untouched previous copy (observe_mid).
-__global__ __launch_bounds__(256) void k_observe_mid(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ E,
-                                                     const float *__restrict__ H, const float *__restrict__ psi,
-                                                     uint32_t s_off, Cap2 cp) {
-    observe_mid(g, a, f, E, H, psi, s_off, blockIdx.x, cp);
+// Both steps' observers of a two-step pass (observe_pair): the first step exists only inside the
+// pass, so its port and probe observers recompute what they read from the untouched previous
+// copy (observe_mid) and its DFT observers read what the pass captured.
+__global__ __launch_bounds__(256) void k_observe_pair(Geo g, ObsArgs a, FixArgs f, const float *__restrict__ Eo,
+                                                      const float *__restrict__ Ho, const float *__restrict__ psio,
+                                                      const float *__restrict__ En, const float *__restrict__ Hn,
+                                                      uint32_t s_off, Cap2 cp) {
+    observe_pair(g, a, f, Eo, Ho, psio, En, Hn, s_off, blockIdx.x, cp);
 }
 
 // ---- energy: sum E^2 + (376.73 H)^2 (cpu.rs:417-421, fdtd.wgsl:547-568) -----------------
@@ -1058,8 +1060,8 @@ void launch_pair(Session &s, uint32_t s_off) {
     if (s.fix_blocks) hipLaunchKernelGGL(k_fix, dim3(s.fix_blocks), dim3(256), 0, st, g, f2, En, Eo, s_off + 1);
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 2), st));
     if (s.obs_blocks && s.observe) {
-        hipLaunchKernelGGL(k_observe_mid, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, s_off, cp);
-        hipLaunchKernelGGL(k_observe, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, En, Hn, s_off + 1);
+        hipLaunchKernelGGL(k_observe_pair, dim3(s.obs_blocks), dim3(256), 0, st, g, s.obs, s.fix, Eo, Ho, Po, En, Hn,
+                           s_off, cp);
     }
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 3), st));
     FG_CHECK(hipGetLastError());
