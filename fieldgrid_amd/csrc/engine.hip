@@ -245,6 +245,19 @@ __global__ __launch_bounds__(256) void k_observe_pair(Geo g, ObsArgs a, FixArgs 
     observe_pair(g, a, f, Eo, Ho, psio, En, Hn, s_off, blockIdx.x, cp);
 }
 
+// The largest coefficient-set index of the per-edge index table (u32 words: one index when
+// wide, two u16 indices when narrow), into out[0] (zeroed before).
+__global__ __launch_bounds__(256) void k_index_max(const uint32_t *__restrict__ idx, uint64_t words, bool wide,
+                                                   uint32_t *__restrict__ out) {
+    uint32_t m = 0;
+    for (uint64_t q = uint64_t(blockIdx.x) * 256 + threadIdx.x; q < words; q += uint64_t(gridDim.x) * 256) {
+        const uint32_t w = idx[q];
+        m = max(m, wide ? w : max(w & 0xFFFFu, w >> 16));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
 // ---- energy: sum E^2 + (376.73 H)^2 (cpu.rs:417-421, fdtd.wgsl:547-568) -----------------
 
 __global__ __launch_bounds__(256) void k_energy(const float *__restrict__ E, const float *__restrict__ H, uint64_t total,
@@ -281,6 +294,11 @@ static void validate(const fg_problem *P, uint32_t driven, const fg_extras *X) {
     if (!P->axes || !P->coef_sets || !P->coef_index || P->n_sets == 0) bad("missing axes or coefficient tables");
     if (P->n_ports && driven >= P->n_ports) bad("driven port " + std::to_string(driven) + " does not exist");
     if (X->nf > FG_MAX_FREQS) bad("at most 4 DFT frequencies");
+    // every table with entries needs its pointer
+    if ((P->n_ports && !P->ports) || (P->n_port_edges && !P->port_edges) || (P->n_loops && !P->loops) ||
+        (P->n_sources && !P->sources) || (P->n_inductors && !P->inductors) ||
+        (P->n_debye && (!P->debye_edges || !P->debye_table)) || (X->n_probes && !X->probes))
+        bad("a table with entries is NULL");
     if (X->plane_k >= int64_t(P->n[2])) bad("plane_k lies outside the grid");
     if (X->n_patches && !X->ntff) bad("NTFF patch table is NULL");
     for (uint64_t q = 0; q < X->n_patches; q++)
@@ -541,6 +559,21 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     }
     s.coef_index.upload(P->coef_index, P->coef_wide ? 3 * nn : (3 * nn + 1) / 2, st);
     s.coef_sets.upload(P->coef_sets, 2 * size_t(P->n_sets), st);
+    {
+        // every index must name a set: the kernels read sets[index] unchecked
+        DevBuf<uint32_t> top;
+        top.alloc(1);
+        top.zero(st);
+        const uint64_t words = s.coef_index.n;
+        const uint32_t blocks = uint32_t(std::min<uint64_t>((words + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_index_max, dim3(blocks), dim3(256), 0, st, s.coef_index.p, words, bool(P->coef_wide), top.p);
+        uint32_t m = 0;
+        FG_CHECK(hipMemcpyAsync(&m, top.p, 4, hipMemcpyDeviceToHost, st));
+        FG_CHECK(hipStreamSynchronize(st));
+        if (m >= P->n_sets)
+            throw HipError{"coefficient-set index " + std::to_string(m) + " is past the " + std::to_string(P->n_sets) +
+                               "-entry set table", FG_INVALID};
+    }
     // axes: AoS rows of 10 -> SoA per axis (8 float fields + 2 int slots)
     const uint32_t nmax = std::max({P->n[0], P->n[1], P->n[2]});
     std::vector<float> axf(3 * 8 * size_t(nmax), 0.f);
@@ -550,6 +583,10 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         for (uint32_t i = 0; i < P->n[a]; i++, row++) {
             const float *r = P->axes + 10 * size_t(row);
             for (int f = 0; f < 8; f++) axf[(size_t(a) * 8 + f) * nmax + i] = r[f];
+            // slots address the CPML psi slabs: -1 or a whole number below 2 pml
+            for (int f = 8; f < 10; f++)
+                if (!(r[f] >= -1.f && r[f] < 2.f * float(P->pml)) || r[f] != float(int(r[f])))
+                    throw HipError{"an axes-table CPML slot is not -1 or below 2 pml", FG_INVALID};
             axs[(size_t(a) * 2 + 0) * nmax + i] = int(r[8]);
             axs[(size_t(a) * 2 + 1) * nmax + i] = int(r[9]);
         }

@@ -1,0 +1,83 @@
+"""fg_run refuses malformed problems with FG_INVALID and a message, before any kernel reads them.
+
+The product builder never makes these tables; a caller that packs its own (the C-ABI takes plain
+pointers) could. Each case copies a valid packed problem and breaks one thing: a
+coefficient-set index past the set table (the kernels read sets[index] unchecked), a CPML slot
+that is not -1 or below 2 pml (slots become psi buffer offsets), a table with entries but a NULL
+pointer.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from fieldgrid_amd import abi
+from fieldgrid_amd import fdtd as F
+from fieldgrid_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _copy(p):
+    q = abi.fg_problem()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(p))
+    return q
+
+
+def _run_with(gpu, sim, problem):
+    sim._problem = (0, problem)  # the run reads the packed problem from the Sim's cache
+    try:
+        F.run(gpu, sim, 0, W.plain_pulse(), F.Extras(max_steps=4, decay_db=math.inf))
+    finally:
+        sim._problem = None
+
+
+def _expect_invalid(gpu, sim, problem, text):
+    with pytest.raises(abi.FieldgridError) as e:
+        _run_with(gpu, sim, problem)
+    print(f"\n  refused: {e.value}")
+    assert e.value.status == abi.FG_INVALID and text in str(e.value)
+
+
+@pytest.mark.parametrize("wide", [False, True], ids=["u16-index", "u32-index"])
+def test_a_coefficient_index_past_the_set_table_is_refused(gpu, wide):
+    sim, _ = W.plain(nodes=24, pml=4)
+    p = sim.problem(narrow=1 if wide else 0)  # a narrow limit of 1 set makes the index u32
+    nn = int(np.prod(sim.dims()))
+    assert bool(p.coef_wide) == wide
+    words = 3 * nn if p.coef_wide else (3 * nn + 1) // 2
+    idx = np.ctypeslib.as_array(p.coef_index, shape=(words,)).copy()
+    bad = idx.copy()
+    if p.coef_wide:
+        bad[words // 2] = p.n_sets
+    else:
+        bad[words // 2] = (bad[words // 2] & 0xFFFF) | (p.n_sets << 16)
+    q = _copy(p)
+    q.coef_index = bad.ctypes.data_as(C.POINTER(C.c_uint32))
+    _expect_invalid(gpu, sim, q, "is past the")
+    # the untouched table still runs
+    q.coef_index = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+    _run_with(gpu, sim, q)
+
+
+def test_a_cpml_slot_out_of_range_is_refused(gpu):
+    sim, _ = W.plain(nodes=24, pml=4)
+    p = sim.problem()
+    rows = sum(sim.dims())
+    axes = np.ctypeslib.as_array(p.axes, shape=(rows * 10,)).copy()
+    for bad_slot in (2.0 * sim.pml, 1.5, float("nan")):
+        ax = axes.copy()
+        ax[8] = bad_slot  # slot_e of node 0 on axis 0
+        q = _copy(p)
+        q.axes = ax.ctypes.data_as(C.POINTER(C.c_float))
+        _expect_invalid(gpu, sim, q, "CPML slot")
+
+
+def test_a_table_with_entries_and_no_pointer_is_refused(gpu):
+    sim, _ = W.plain(nodes=24, pml=4)
+    p = sim.problem()
+    assert p.n_port_edges > 0
+    q = _copy(p)
+    q.port_edges = None
+    _expect_invalid(gpu, sim, q, "is NULL")
