@@ -751,6 +751,29 @@ def finish(gpu: Gpu, rec: Record, dt: float, freqs: Sequence[float], f0: float) 
     return Outcome(list(freqs), z[:-1], complex(z[-1]), float(0.5 * (V[0, -1] * np.conj(I[0, -1])).real))
 
 
+def s_parameters(spectra, z0=50.0) -> np.ndarray:
+    """The scattering matrix of an N-port from a port sweep: N runs of one problem, run d
+    driving port d, spectra[d] = (V, I) of shape (N, nf) as run_batch or port_spectra give them.
+    Power waves at a real reference impedance z0 (scalar, or one per port):
+    a = (V + z0 I) / (2 sqrt(z0)), b = (V - z0 I) / (2 sqrt(z0)), with I flowing into the
+    structure, as finish() takes it (p_in = Re(V I*) / 2 > 0 at the driven port); then
+    S = B A^-1 at each frequency, the columns of A and B being the runs. With z0 equal to the
+    ports' own resistances only the driven port carries an incident wave. No counterpart in the
+    reference, whose sims-rf networks take S-parameters from elsewhere. Returns (nf, N, N)."""
+    runs = len(spectra)
+    V = np.stack([np.asarray(v, complex) for v, _ in spectra], axis=2)  # (port, f, run)
+    I = np.stack([np.asarray(i, complex) for _, i in spectra], axis=2)
+    if V.shape[0] != runs or I.shape != V.shape:
+        raise ValueError(f"an {V.shape[0]}-port needs {V.shape[0]} runs, one per driven port, of matching spectra")
+    z = np.broadcast_to(np.asarray(z0, float), (runs,)).reshape(runs, 1, 1)
+    if np.any(z <= 0):
+        raise ValueError("reference impedances must be positive")
+    A = (V + z * I) / (2.0 * np.sqrt(z))
+    B = (V - z * I) / (2.0 * np.sqrt(z))
+    A, B = np.moveaxis(A, 1, 0), np.moveaxis(B, 1, 0)  # (f, port, run)
+    return np.linalg.solve(np.swapaxes(A, 1, 2), np.swapaxes(B, 1, 2)).swapaxes(1, 2)
+
+
 class Session:
     """A resident run (fg_session_*): upload once, step, read. Used by benchmarks."""
 
