@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -28,6 +29,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fieldgrid_batch.h"
@@ -164,10 +166,11 @@ struct Batch {
         graph = nullptr;
     }
     ~Batch() {
-        if (stream) (void)hipStreamSynchronize(stream);  // an error may leave work in flight
+        const bool idle = stream && hipStreamSynchronize(stream) == hipSuccess;  // an error may leave work in flight
         drop_graph();
         for (auto e : ev) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (idle) give_stream(dev, stream);
+        else if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -577,6 +580,42 @@ static void spectra(Batch &b, uint32_t count, fg_spectra *sp) {
     }
 }
 
+// The runs' sessions, built on up to 8 host threads. A session's set-up is mostly host work
+// (tables built, copied into pinned memory) and waits on its own stream, so the set-ups of
+// different runs overlap. The first error is rethrown once every thread has stopped.
+static void make_sessions(Batch &b, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
+                          const fg_pulse *pulses, const fg_extras *const *extras) {
+    std::vector<fg_extras> xs(count);
+    for (uint32_t r = 0; r < count; r++) xs[r] = *extras[r], xs[r].engine = FG_ENGINE_FUSED;
+    b.runs.resize(count);
+    std::atomic<uint32_t> next{0};
+    std::mutex m;
+    std::exception_ptr first;
+    auto work = [&] {
+        try {
+            FG_CHECK(hipSetDevice(b.dev->ordinal));
+            for (uint32_t r; (r = next.fetch_add(1)) < count;)
+                b.runs[r].reset(session_new(b.dev, problems[r], driven[r], &pulses[r], &xs[r]));
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(m);
+            if (!first) first = std::current_exception();
+            next.store(count);
+        }
+    };
+    const uint32_t threads = std::min({count, std::max(1u, std::thread::hardware_concurrency()), 8u});
+    std::vector<std::thread> pool;
+    for (uint32_t q = 1; q < threads; q++) {
+        try {
+            pool.emplace_back(work);
+        } catch (const std::system_error &) {
+            break;  // fewer threads
+        }
+    }
+    work();
+    for (auto &t : pool) t.join();
+    if (first) std::rethrow_exception(first);
+}
+
 }  // namespace fg
 
 extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
@@ -589,7 +628,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         Batch b;
         b.dev = dev;
         FG_CHECK(hipSetDevice(dev->ordinal));
-        FG_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        b.stream = take_stream(dev);
         b.xcds = dev->cus >= 64 ? uint32_t(dev->cus / 32) : 1u;  // gfx950: 32 CUs per XCD
         b.cus = uint32_t(std::max(dev->cus, 1));
         std::vector<BatchRun> table(count);
@@ -599,10 +638,10 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             if (!problems[r] || !extras[r]) throw HipError{"fg_run_batch: NULL problem or extras", FG_INVALID};
             if (problems[r]->n_sheets || problems[r]->n_wires)
                 throw HipError{"fg_run_batch: resistive sheets and thin wires run through fg_run", FG_UNSUPPORTED};
-            fg_extras x = *extras[r];
-            x.engine = FG_ENGINE_FUSED;
-            b.runs.emplace_back(session_new(dev, problems[r], driven[r], &pulses[r], &x));
-            Session &s = *b.runs.back();
+        }
+        make_sessions(b, count, problems, driven, pulses, extras);
+        for (uint32_t r = 0; r < count; r++) {
+            Session &s = *b.runs[r];
             BatchRun &t = table[r];
             t.g = s.geo;
             for (int c = 0; c < 2; c++) t.E[c] = s.E[c].p, t.H[c] = s.H[c].p, t.psi[c] = s.psi[c].p;

@@ -33,10 +33,12 @@ namespace fg {
 
 thread_local std::string g_error;
 thread_local Stager *t_stager = nullptr;
+thread_local DevArena *t_arena = nullptr;
 
 namespace {
 std::mutex g_pin_mutex;
-std::vector<char *> g_pin_free;  // pinned chunks of Stager::CHUNK bytes not in use
+std::vector<char *> g_pin_free;         // pinned chunks of Stager::CHUNK bytes not in use
+std::vector<Stager::Slot> g_slot_free;  // pinned ring slots of Stager::SLOT bytes not in use
 }  // namespace
 
 hipStream_t take_stream(fg_device *dev) {
@@ -62,14 +64,17 @@ Stager::Stager(hipStream_t s) : stream_(s), prev_(t_stager) { t_stager = this; }
 
 Stager::~Stager() {
     t_stager = prev_;
-    if (chunks_.empty()) return;
-    (void)hipStreamSynchronize(stream_);  // the DMAs have read the chunks
+    if (chunks_.empty() && ring_.empty()) return;
+    // the DMAs have read the pinned memory; if that cannot be known, it stays out of the pools
+    if (hipStreamSynchronize(stream_) != hipSuccess) return;
     std::lock_guard<std::mutex> lock(g_pin_mutex);
     for (char *c : chunks_) g_pin_free.push_back(c);
+    for (Slot sl : ring_) sl.busy = false, g_slot_free.push_back(sl);
 }
 
 bool Stager::copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
-    if (s != stream_ || bytes > MAX_BYTES) return false;
+    if (s != stream_) return false;
+    if (bytes > MAX_BYTES) return copy_large(dst, src, bytes);
     const size_t at = (used_ + 255) & ~size_t(255);
     if (at + bytes > CHUNK) {
         char *c = nullptr;
@@ -86,6 +91,49 @@ bool Stager::copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
     FG_CHECK(hipMemcpyAsync(dst, chunks_.back() + at, bytes, hipMemcpyHostToDevice, s));
     used_ = at + bytes;
     return true;
+}
+
+bool Stager::copy_large(void *dst, const void *src, size_t bytes) {
+    if (ring_.empty()) {
+        std::lock_guard<std::mutex> lock(g_pin_mutex);
+        while (ring_.size() < RING && !g_slot_free.empty()) ring_.push_back(g_slot_free.back()), g_slot_free.pop_back();
+    }
+    while (ring_.size() < RING) {
+        Slot sl{nullptr, nullptr, false};
+        FG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&sl.host), SLOT, hipHostMallocDefault));
+        const hipError_t e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e != hipSuccess) (void)hipHostFree(sl.host);
+        FG_CHECK(e);
+        ring_.push_back(sl);
+    }
+    for (size_t at = 0; at < bytes; at += SLOT) {
+        Slot &sl = ring_[next_++ % RING];
+        if (sl.busy) FG_CHECK(hipEventSynchronize(sl.done));  // its previous DMA has read it
+        const size_t len = std::min(SLOT, bytes - at);
+        std::memcpy(sl.host, static_cast<const char *>(src) + at, len);
+        FG_CHECK(hipMemcpyAsync(static_cast<char *>(dst) + at, sl.host, len, hipMemcpyHostToDevice, stream_));
+        FG_CHECK(hipEventRecord(sl.done, stream_));
+        sl.busy = true;
+    }
+    return true;
+}
+
+DevArena::~DevArena() {
+    for (void *c : chunks_) (void)hipFree(c);
+}
+
+void *DevArena::take(size_t bytes) {
+    if (bytes > MAX_BYTES) return nullptr;
+    const size_t at = (used_ + 255) & ~size_t(255);
+    if (at + bytes > CHUNK) {
+        void *c = nullptr;
+        FG_CHECK(hipMalloc(&c, CHUNK));
+        chunks_.push_back(c);
+        used_ = 0;
+        return take(bytes);
+    }
+    used_ = at + bytes;
+    return static_cast<char *>(chunks_.back()) + at;
 }
 
 int set_error(int status, const std::string &msg) {
@@ -537,7 +585,8 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     }
     s.stream = take_stream(dev);
     hipStream_t st = s.stream;
-    Stager stage(st);  // the tables below go up through pinned memory
+    Stager stage(st);               // the tables below go up through pinned memory
+    ArenaScope arena_scope(s.arena);  // and the small ones into the session's arena
     // concurrent block classes: on a 16.7 M-node grid the classes' last rounds overlap for ~10 %;
     // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves
     const char *cc = getenv("FIELDGRID_STREAMS");
@@ -549,13 +598,19 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         }
         FG_CHECK(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
     }
-    for (int c = 0; c < copies; c++) {
-        s.E[c].alloc(3 * nn);
-        s.H[c].alloc(3 * nn);
-        s.psi[c].alloc(std::max<uint64_t>(psi_total, 1));
-        s.E[c].zero(st);
-        s.H[c].zero(st);
-        s.psi[c].zero(st);
+    {
+        // E, H and psi of every copy in one allocation (one hipMalloc/hipFree instead of six),
+        // each array at a 256-byte boundary, all of it zeroed
+        auto round = [](uint64_t bytes) { return (bytes + 255) & ~uint64_t(255); };
+        const uint64_t eb = round(12 * nn), pb = round(4 * std::max<uint64_t>(psi_total, 1));
+        s.field_mem.alloc(copies * (2 * eb + pb));
+        s.field_mem.zero(st);
+        char *at = s.field_mem.p;
+        for (int c = 0; c < copies; c++) {
+            s.E[c].carve(reinterpret_cast<float *>(at), 3 * nn), at += eb;
+            s.H[c].carve(reinterpret_cast<float *>(at), 3 * nn), at += eb;
+            s.psi[c].carve(reinterpret_cast<float *>(at), std::max<uint64_t>(psi_total, 1)), at += pb;
+        }
     }
     s.coef_index.upload(P->coef_index, P->coef_wide ? 3 * nn : (3 * nn + 1) / 2, st);
     s.coef_sets.upload(P->coef_sets, 2 * size_t(P->n_sets), st);

@@ -32,47 +32,103 @@ struct HipError {
 
 constexpr int ENERGY_BLOCKS = 1024;
 
-// Small host-to-device copies through pinned host memory. From pageable memory the runtime
-// stages each copy itself and returns only once it is done (~0.1 ms per copy, ~20 per session
-// set-up); from pinned memory a copy is an asynchronous DMA. While a Stager is installed on
-// this thread, DevBuf::upload on the Stager's stream copies through it: the source goes into a
-// pinned chunk (hipHostMalloc, from a process-wide pool that keeps its chunks until the process
-// ends) and the DMA reads it from there. The destructor synchronizes the stream before the
-// chunks go back to the pool.
+// Host-to-device copies through pinned host memory. From pageable memory the runtime stages
+// each copy itself and returns only once it is done (~0.1 ms per small copy, ~20 per session
+// set-up; ~3 GB/s for a large one); from pinned memory a copy is an asynchronous DMA. While a
+// Stager is installed on this thread, DevBuf::upload on the Stager's stream copies through it:
+//   - up to MAX_BYTES: the source goes into a pinned chunk of CHUNK bytes, shared with the next
+//     small copies, and the DMA reads it from there;
+//   - larger: in pieces of SLOT bytes through a ring of RING pinned slots, the CPU filling one
+//     slot while the DMA engine reads the others (each slot's event says when its last DMA is
+//     done).
+// Chunks and slots come from process-wide pools (hipHostMalloc) that keep them until the
+// process ends. The destructor synchronizes the stream before they go back to the pools.
 class Stager {
   public:
     explicit Stager(hipStream_t s);
     ~Stager();
     Stager(const Stager &) = delete;
     Stager &operator=(const Stager &) = delete;
-    // enqueue dst <- src on stream s through a pinned chunk; false when not staged (another
-    // stream, or more than MAX_BYTES): the caller copies directly
+    // enqueue dst <- src on stream s through pinned memory; false when s is another stream (the
+    // caller copies directly)
     bool copy(void *dst, const void *src, size_t bytes, hipStream_t s);
-    static constexpr size_t CHUNK = 1u << 20, MAX_BYTES = 256u << 10;
+    static constexpr size_t CHUNK = 1u << 20, MAX_BYTES = 256u << 10, SLOT = 4u << 20, RING = 4;
+    struct Slot {
+        char *host;
+        hipEvent_t done;
+        bool busy;  // a DMA from this slot was enqueued since the slot was last waited for
+    };
 
   private:
+    bool copy_large(void *dst, const void *src, size_t bytes);
     hipStream_t stream_;
     Stager *prev_;
     std::vector<char *> chunks_;
     size_t used_ = CHUNK;  // bytes used in chunks_.back()
+    std::vector<Slot> ring_;
+    size_t next_ = 0;  // the ring slot the next piece goes through (mod RING)
 };
 extern thread_local Stager *t_stager;
+
+// Device memory for one session's small tables, carved at 256-byte alignment from chunks of
+// CHUNK bytes: a session makes one hipMalloc and one hipFree per chunk instead of one per table
+// (each call costs tens of microseconds, and a session has about 40 tables). Requests above
+// MAX_BYTES are refused (take returns nullptr) and get their own hipMalloc. The chunks are freed
+// with the arena, so it must outlive every buffer carved from it.
+class DevArena {
+  public:
+    DevArena() = default;
+    ~DevArena();
+    DevArena(const DevArena &) = delete;
+    DevArena &operator=(const DevArena &) = delete;
+    void *take(size_t bytes);
+    static constexpr size_t CHUNK = 2u << 20, MAX_BYTES = 256u << 10;
+
+  private:
+    std::vector<void *> chunks_;
+    size_t used_ = CHUNK;  // bytes used in chunks_.back()
+};
+// While an ArenaScope is alive on this thread, DevBuf::alloc carves small buffers from its arena.
+extern thread_local DevArena *t_arena;
+class ArenaScope {
+  public:
+    explicit ArenaScope(DevArena &a) : prev_(t_arena) { t_arena = &a; }
+    ~ArenaScope() { t_arena = prev_; }
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+
+  private:
+    DevArena *prev_;
+};
 
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool pooled = false;  // p was carved from a DevArena or another buffer, which frees it
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
+    ~DevBuf() { release(); }
+    void release() {
+        if (p && !pooled) (void)hipFree(p);
+        p = nullptr;
+        pooled = false;
     }
     void alloc(size_t count) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
+        release();
         n = count;
-        if (count) FG_CHECK(hipMalloc(&p, count * sizeof(T)));
+        if (!count) return;
+        if (t_arena && (p = static_cast<T *>(t_arena->take(count * sizeof(T))))) {
+            pooled = true;
+            return;
+        }
+        FG_CHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    // view count elements at `at`, memory that another buffer owns and frees
+    void carve(T *at, size_t count) {
+        release();
+        p = at, n = count, pooled = true;
     }
     void zero(hipStream_t s) {
         if (n) FG_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
@@ -106,6 +162,7 @@ void give_stream(fg_device *dev, hipStream_t s);
 
 struct Session {
     fg_device *dev;
+    DevArena arena;  // the small tables below; declared before them, so destroyed after them
     hipStream_t stream = nullptr;
     Geo geo{};
     FixArgs fix{};
@@ -126,6 +183,7 @@ struct Session {
     double kernel_ms[3] = {0, 0, 0};  // step kernel(s), fix, observe
     uint64_t kernel_count = 0;
     // buffers
+    DevBuf<char> field_mem;  // holds E, H and psi; declared before them, so freed after them
     DevBuf<float> E[2], H[2], psi[2];
     DevBuf<uint32_t> coef_index, step;
     DevBuf<float> coef_sets, axes, dtab, dstate, dold, ind_i, series, plane, ntff, probes, pulse, phase;
