@@ -226,14 +226,16 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     }
     for (int q = 0; q < N_LISTS; q++) {
         xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
-        b.lists[q].upload(L[q].data(), L[q].size(), b.stream);
+        b.lists[q].refill(L[q].data(), L[q].size(), b.stream);
         b.list_n[q] = uint32_t(L[q].size());
         xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
-        b.lists2[q].upload(L2[q].data(), L2[q].size(), b.stream);
+        b.lists2[q].refill(L2[q].data(), L2[q].size(), b.stream);
         b.list2_n[q] = uint32_t(L2[q].size());
     }
-    b.fix_list.upload(fl.data(), fl.size(), b.stream);
-    b.obs_list.upload(ol.data(), ol.size(), b.stream);
+    // the lists are rewritten in place (stream-ordered after the launches that read them)
+    // as runs finish, so the batch allocates them once
+    b.fix_list.refill(fl.data(), fl.size(), b.stream);
+    b.obs_list.refill(ol.data(), ol.size(), b.stream);
     b.n_fix = uint32_t(fl.size());
     b.n_obs = uint32_t(ol.size());
     FG_CHECK(hipStreamSynchronize(b.stream));

@@ -104,8 +104,8 @@ class ArenaScope {
 template <class T>
 struct DevBuf {
     T *p = nullptr;
-    size_t n = 0;
-    bool pooled = false;  // p was carved from a DevArena or another buffer, which frees it
+    size_t n = 0, cap = 0;  // elements in use, elements allocated
+    bool pooled = false;    // p was carved from a DevArena or another buffer, which frees it
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -114,10 +114,11 @@ struct DevBuf {
         if (p && !pooled) (void)hipFree(p);
         p = nullptr;
         pooled = false;
+        cap = 0;
     }
     void alloc(size_t count) {
         release();
-        n = count;
+        n = cap = count;
         if (!count) return;
         if (t_arena && (p = static_cast<T *>(t_arena->take(count * sizeof(T))))) {
             pooled = true;
@@ -128,13 +129,25 @@ struct DevBuf {
     // view count elements at `at`, memory that another buffer owns and frees
     void carve(T *at, size_t count) {
         release();
-        p = at, n = count, pooled = true;
+        p = at, n = cap = count, pooled = true;
     }
     void zero(hipStream_t s) {
         if (n) FG_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
     }
     void upload(const T *src, size_t count, hipStream_t s) {
         alloc(count);
+        copy_in(src, count, s);
+    }
+    // upload into the present allocation when it holds count elements (the copy is ordered on
+    // stream s after the work already there), else into a new one
+    void refill(const T *src, size_t count, hipStream_t s) {
+        if (p && count <= cap) n = count;
+        else alloc(count);
+        copy_in(src, count, s);
+    }
+
+  private:
+    void copy_in(const T *src, size_t count, hipStream_t s) {
         if (!count || (t_stager && t_stager->copy(p, src, count * sizeof(T), s))) return;
         FG_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
