@@ -81,3 +81,34 @@ def test_a_table_with_entries_and_no_pointer_is_refused(gpu):
     q = _copy(p)
     q.port_edges = None
     _expect_invalid(gpu, sim, q, "is NULL")
+
+
+def test_a_bad_run_in_a_batch_is_refused_and_the_batch_runs_once_fixed(gpu):
+    """The batch builds its runs' device state on several host threads; an error in one of them
+    (here a coefficient index past the set table in run 6 of 10) comes back as that run's
+    FG_INVALID once every thread has stopped, and nothing of the failed batch is left behind."""
+    runs = W.sweep_runs(count=10, lo=24, hi=40, pml=4, nboxes=2, ndebye=1, steps=(20, 40), nfreq=4)
+    sim = runs[6].sim
+    p = sim.problem()
+    nn = int(np.prod(sim.dims()))
+    words = 3 * nn if p.coef_wide else (3 * nn + 1) // 2
+    bad = np.ctypeslib.as_array(p.coef_index, shape=(words,)).copy()
+    if p.coef_wide:
+        bad[words // 3] = p.n_sets
+    else:
+        bad[words // 3] = (bad[words // 3] & 0xFFFF) | (p.n_sets << 16)
+    q = _copy(p)
+    q.coef_index = bad.ctypes.data_as(C.POINTER(C.c_uint32))
+    tune = F.Tune(tj=14, seg=16, chunk=1)
+    sim._problem = (0, q)
+    try:
+        with pytest.raises(abi.FieldgridError) as e:
+            F.run_batch(gpu, runs, tune=tune)
+    finally:
+        sim._problem = None
+    print(f"\n  refused: {e.value}")
+    assert e.value.status == abi.FG_INVALID and "is past the" in str(e.value)
+    recs, _ = F.run_batch(gpu, runs, tune=tune)
+    for r in (0, 6, 9):
+        one = F.run(gpu, runs[r].sim, runs[r].driven, runs[r].pulse, runs[r].extras)
+        assert np.array_equal(recs[r].series, one.series), f"run {r}"
