@@ -62,7 +62,9 @@ typedef struct fg_spectra {
    calling thread whenever any run reaches a decay check, with the batch's step count and the
    smallest decay among the runs still going; returning 0 cancels the whole batch. Resistive
    sheets and thin wires give FG_UNSUPPORTED here (fg_run runs them). A run that goes unstable fails the
-   batch with FG_UNSTABLE and "run <r>: the FDTD run went unstable". */
+   batch with FG_UNSTABLE and "run <r>: the FDTD run went unstable". The runs' device state is
+   built on up to 8 host threads that end before the batch steps; an invalid problem fails the
+   batch with its own status and message, as fg_run would. */
 int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *problems, const uint32_t *driven,
                  const fg_pulse *pulses, const fg_extras *const *extras, fg_tune *tune, fg_progress_fn progress,
                  void *user, fg_record *records, fg_spectra *spectra);
