@@ -641,6 +641,20 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             if (problems[r]->n_sheets || problems[r]->n_wires)
                 throw HipError{"fg_run_batch: resistive sheets and thin wires run through fg_run", FG_UNSUPPORTED};
         }
+        {
+            // the runs' sessions are built concurrently, so each one's own capacity check cannot
+            // see the others': check their sum first
+            double need = 0.0;
+            for (uint32_t r = 0; r < count; r++) need += session_bytes(problems[r], true);
+            size_t free_b = 0, total_b = 0;
+            FG_CHECK(hipMemGetInfo(&free_b, &total_b));
+            if (need > double(free_b)) {
+                char msg[160];
+                std::snprintf(msg, sizeof msg, "the batch's %u runs need %.1f GB of device memory; %.1f GB are free",
+                              count, need / 1e9, double(free_b) / 1e9);
+                throw HipError{msg, FG_TOO_LARGE};
+            }
+        }
         make_sessions(b, count, problems, driven, pulses, extras);
         for (uint32_t r = 0; r < count; r++) {
             Session &s = *b.runs[r];

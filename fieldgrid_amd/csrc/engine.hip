@@ -407,6 +407,13 @@ static uint64_t psi_len(const uint32_t n[3], int a, uint32_t pml) {
     return m[0] * m[1] * m[2];
 }
 
+double session_bytes(const fg_problem *P, bool fused) {
+    const double nn = double(P->n[0]) * P->n[1] * P->n[2], pml = P->pml;
+    double psi = 0.0;  // 12 psi arrays, two per (component, other axis); a slab of 2 pml along it
+    for (int a = 0; a < 3; a++) psi += pml ? 4.0 * (nn / P->n[a]) * 2.0 * pml : 0.0;
+    return (fused ? 2.0 : 1.0) * (24.0 * nn + 4.0 * psi) + (P->coef_wide ? 12.0 : 6.0) * nn;
+}
+
 // Split the march blocks into those whose reach (their own nodes plus the one-node halo on
 // every side) touches a CPML slab position and those that do not.
 void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]) {
@@ -576,10 +583,9 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
                 if (a != c) psi_total += psi_len(P->n, a, P->pml);
             }
     const int copies = s.fused ? 2 : 1;
-    const uint64_t need = uint64_t(copies) * (24 * nn + 4 * psi_total) + (P->coef_wide ? 12 : 6) * nn;
     size_t free_b = 0, total_b = 0;
     FG_CHECK(hipMemGetInfo(&free_b, &total_b));
-    if (need > free_b) {
+    if (session_bytes(P, s.fused) > double(free_b)) {
         const uint64_t cells = uint64_t(P->n[0] - 1) * (P->n[1] - 1) * (P->n[2] - 1);
         throw HipError{std::to_string(cells) + " cells is more than this GPU can hold", FG_TOO_LARGE};
     }

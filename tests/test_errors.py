@@ -112,3 +112,15 @@ def test_a_bad_run_in_a_batch_is_refused_and_the_batch_runs_once_fixed(gpu):
     for r in (0, 6, 9):
         one = F.run(gpu, runs[r].sim, runs[r].driven, runs[r].pulse, runs[r].extras)
         assert np.array_equal(recs[r].series, one.series), f"run {r}"
+
+
+def test_a_batch_larger_than_the_device_is_refused_before_it_allocates(gpu):
+    """The runs' sessions are built concurrently, so the batch checks the sum of their device
+    bytes against the free memory first: 700 runs of one 200^3 problem (~0.5 GB each) ask for
+    far more than one GPU holds."""
+    sim, _ = W.plain(nodes=200, pml=8)
+    runs = [F.BatchRun(sim, 0, W.plain_pulse(), F.Extras(max_steps=10, decay_db=math.inf), [])] * 700
+    with pytest.raises(abi.FieldgridError) as e:
+        F.run_batch(gpu, runs, tune=F.Tune(tj=14, seg=16, chunk=1))
+    print(f"\n  refused: {e.value}")
+    assert e.value.status == abi.FG_TOO_LARGE and "the batch's 700 runs need" in str(e.value)
