@@ -488,8 +488,9 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
                 // where two steps per pass are possible, one step per pass is only tried with
                 // 14-row tiles (8-row tiles re-read more halo and have not won there)
                 if (depth == 1 && tj == 8 && ask_depth && ask_tj) continue;
-                for (uint32_t seg : {32u, 16u, 8u}) {
-                    if (!ask_seg && seg != seg_in) continue;
+                // the caller's segment length is timed as given, also one outside the list
+                const std::vector<uint32_t> segs = ask_seg ? std::vector<uint32_t>{32u, 16u, 8u} : std::vector<uint32_t>{seg_in};
+                for (uint32_t seg : segs) {
                     b.depth = depth, b.tj = tj, b.seg = b.seg_base = b.seg2 = b.seg2_base = seg;
                     plan(b, active);
                     if (!steps) {

@@ -203,3 +203,17 @@ def test_device_port_spectra_and_finish_match_the_direct_sums(gpu):
     assert np.max(np.abs(out.z - z[:-1])) <= 1e-9 * np.max(np.abs(z))
     assert abs(out.z0 - z[-1]) <= 1e-9 * abs(z[-1])
     assert abs(out.p_in - 0.5 * (V3[0, -1] * np.conj(I3[0, -1])).real) <= 1e-9 * abs(V3[0, -1] * I3[0, -1])
+
+
+@pytest.mark.gpu
+def test_a_given_segment_outside_the_tuners_list_is_timed_as_given(gpu):
+    """With the segment length fixed by the caller (20 planes, not one of the tuner's 32/16/8)
+    and the depth left to the tuner, the tuner times that length: a finite step time, and the
+    records still equal single runs."""
+    runs = small_sweep(count=3)
+    tune = F.Tune(seg=20)
+    recs, _ = F.run_batch(gpu, runs, tune=tune)
+    print(f"\n  used {tune.used}, step {tune.step_ms:.3f} ms")
+    assert tune.used[1] == 20 and 0.0 < tune.step_ms < 1e3
+    for q, r in enumerate(runs):
+        _same(recs[q], F.run(gpu, r.sim, r.driven, r.pulse, r.extras), f"run {q}")
