@@ -67,6 +67,8 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_march_batch(const BatchRun *_
 
 constexpr int TJ2 = 12;  // rows per block of the two-step march
 
+// PML -1: the block list mixes the CPML classes, each block's class in the two top bits of its
+// bz; the class instances share one LDS copy, and one launch per pass replaces three.
 template <int PML, bool WIDE, bool LDS_SETS>
 __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun *__restrict__ runs,
                                                                  const BatchBlock *__restrict__ blocks, uint32_t par,
@@ -76,8 +78,16 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun 
     const BatchRun &r = runs[bb.run];
     const uint32_t n = r.fix.step[0] + s_off;
     const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
-    march2<TJ2, PML, WIDE, LDS_SETS, true>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
-                                     r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{});
+    __shared__ March2Lds<TJ2, LDS_SETS, true> lds;
+    const MarchBlock mb{bb.bx, bb.by, bb.bz & 0x3FFFFFFFu};
+    const int cls = PML >= 0 ? PML : int(bb.bz >> 30);
+#define FG_M2(C)                                                                                                    \
+    march2<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u], r.psi[par], \
+                                         r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{})
+    if (cls == 2) FG_M2(2);
+    else if (cls == 1) FG_M2(1);
+    else FG_M2(0);
+#undef FG_M2
 }
 
 __global__ __launch_bounds__(256) void k_fix_batch(const BatchRun *__restrict__ runs, const uint32_t *__restrict__ list,
@@ -226,9 +236,24 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     }
     for (int q = 0; q < N_LISTS; q++) {
         xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
+        xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
+    }
+    // the two-step march runs each key's three classes in one mixed list (k_march2_batch<-1>),
+    // class-tagged, the longest blocks (x/y, then z) first; each class list is padded to a
+    // multiple of the XCD count, so the concatenation keeps its XCD order
+    for (int key = 0; key < 4; key++) {
+        std::vector<BatchBlock> mix;
+        for (int c = 2; c >= 0; c--)
+            for (BatchBlock m : L2[c * 4 + key]) {
+                if (m.bx != ~0u) m.bz |= uint32_t(c) << 30;
+                mix.push_back(m);
+            }
+        L2[key].swap(mix);
+        L2[4 + key].clear(), L2[8 + key].clear();
+    }
+    for (int q = 0; q < N_LISTS; q++) {
         b.lists[q].refill(L[q].data(), L[q].size(), b.stream);
         b.list_n[q] = uint32_t(L[q].size());
-        xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
         b.lists2[q].refill(L2[q].data(), L2[q].size(), b.stream);
         b.list2_n[q] = uint32_t(L2[q].size());
     }
@@ -254,14 +279,12 @@ static void launch_list2(Batch &b, int q, uint32_t par, uint32_t s_off) {
 }
 
 static void launch_march2(Batch &b, uint32_t par, uint32_t s_off) {
-    for (int q = 0; q < N_LISTS; q++) {
+    for (int q = 0; q < 4; q++) {  // the mixed lists, one per key (plan)
         if (!b.list2_n[q]) continue;
         switch (q) {
-#define FG_CASE(Q, PML, W, L) \
-    case Q: launch_list2<PML, W, L>(b, q, par, s_off); break;
-            FG_CASE(0, 0, false, false) FG_CASE(1, 0, false, true) FG_CASE(2, 0, true, false) FG_CASE(3, 0, true, true)
-            FG_CASE(4, 1, false, false) FG_CASE(5, 1, false, true) FG_CASE(6, 1, true, false) FG_CASE(7, 1, true, true)
-            FG_CASE(8, 2, false, false) FG_CASE(9, 2, false, true) FG_CASE(10, 2, true, false) FG_CASE(11, 2, true, true)
+#define FG_CASE(Q, W, L) \
+    case Q: launch_list2<-1, W, L>(b, q, par, s_off); break;
+            FG_CASE(0, false, false) FG_CASE(1, false, true) FG_CASE(2, true, false) FG_CASE(3, true, true)
 #undef FG_CASE
         }
     }

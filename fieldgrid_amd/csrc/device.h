@@ -839,22 +839,39 @@ struct Cap2 {
     const uint32_t *plane;
 };
 
+// march2's LDS, declared once by the kernel that calls it, so that a kernel running blocks of
+// several CPML classes (several march2 instances) holds one copy. Row exchanges are
+// double-buffered by plane parity; row w is stored at index w+1 so that rows -1 and ROWS read
+// as zero pads. xr[0] E0 (Ex, Ez) for the row above's step-n H update, xr[1] step-n H (Hx, Hz)
+// for the row below's step-n E update; xr[2], xr[3] the same for step n+1.
+template <int TJ, bool LDS_SETS, bool SIDE>
+struct March2Lds {
+    static constexpr int ROWS = TJ + 4;
+    static constexpr uint32_t CAP_SEGS = SIDE ? 1536 : 1;
+    float xr[4][2][ROWS + 2][2][64];
+    float2 sets[LDS_SETS ? SETS_LDS : 1];
+    float xa[SEG_MAX + 3][10];  // x-axis values of the segment's planes (the CPML ones from [2])
+    uint32_t xsrc[SEG_MAX + 4];
+    int32_t xdeb[SIDE ? (SEG_MAX + 3) * ROWS : 1];
+    uint32_t xcap[SIDE ? SEG_MAX * (TJ + 1) : 1];
+    uint32_t xcoff[SIDE ? SEG_MAX + 1 : 1];
+    uint4 xseg[CAP_SEGS];
+};
+
 // SIDE: the first step has side work (Debye ADE through db, captures through cp; either may be
 // absent, NULL tables)
 template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool SIDE>
-__device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const float *__restrict__ Eo,
+__device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const Geo &g, const MarchBlock mb,
+                                       const float *__restrict__ Eo,
                                        float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
                                        const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
                                        const Src2 &sr, float pl, const FixArgs &fx, const Deb2 &db,
                                        const Cap2 &cp) {
     constexpr int ROWS = TJ + 4;
     constexpr bool PZ = PML >= 1, PXY = PML == 2;
-    // row exchanges, double-buffered by plane parity; row w is stored at index w+1 so that
-    // rows -1 and ROWS read as zero pads. [0] E0 (Ex, Ez) for the row above's step-n H update,
-    // [1] step-n H (Hx, Hz) for the row below's step-n E update; [2], [3] the same for step n+1.
-    __shared__ float xr[4][2][ROWS + 2][2][64];
-    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
-    __shared__ float xa[SEG_MAX + 3][PXY ? 10 : 2];
+    auto &xr = lds.xr;
+    auto &sets = lds.sets;
+    auto &xa = lds.xa;
     const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int k = int(mb.bx) * OWN_K2 + lane - 2;
     const int j = int(mb.by) * TJ + w - 2;
@@ -923,12 +940,12 @@ __device__ __forceinline__ void march2(const Geo &g, const MarchBlock mb, const 
     //   xdeb[(t - tstart) * ROWS + w]: the Debye row slot of (t, row w), -1 outside the grid;
     //   xcap[(t - i0) * (TJ + 1) + r]: cp.start of owned row r of owned plane t (r = TJ: the end),
     //   xseg: the owned planes' capture segments, plane p's at xcoff[p] on, when they all fit.
-    constexpr uint32_t CAP_SEGS = SIDE ? 1536 : 1;
-    __shared__ uint32_t xsrc[SEG_MAX + 4];
-    __shared__ int32_t xdeb[SIDE ? (SEG_MAX + 3) * ROWS : 1];
-    __shared__ uint32_t xcap[SIDE ? SEG_MAX * (TJ + 1) : 1];
-    __shared__ uint32_t xcoff[SIDE ? SEG_MAX + 1 : 1];
-    __shared__ uint4 xseg[CAP_SEGS];
+    constexpr uint32_t CAP_SEGS = March2Lds<TJ, LDS_SETS, SIDE>::CAP_SEGS;
+    auto &xsrc = lds.xsrc;
+    auto &xdeb = lds.xdeb;
+    auto &xcap = lds.xcap;
+    auto &xcoff = lds.xcoff;
+    auto &xseg = lds.xseg;
     for (uint32_t t = w * 64 + lane; t + tstart <= i1 + 1; t += 64 * ROWS) xsrc[t] = sr.start[min(tstart + t, g.n0)];
     bool seg_lds = false;
     if constexpr (SIDE) {

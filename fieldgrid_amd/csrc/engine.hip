@@ -280,7 +280,32 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
     if (mb.bx == ~0u) return;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    march2<TJ, PML, WIDE, LDS_SETS, DEB>(g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
+    march2<TJ, PML, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+}
+
+// The same for a block list that mixes the CPML classes: the class is in the two top bits of
+// the block's bz (march_classes2's out[c] tagged c), each block runs its class's march2, and the
+// instances share one LDS copy. One launch replaces the three class launches of a pass where
+// those would run one after another (no side streams), so a small grid's pass lasts about one
+// block instead of three.
+template <int TJ, bool WIDE, bool LDS_SETS, bool DEB>
+__global__ __launch_bounds__(64 * (TJ + 4)) void k_march2_mix(Geo g, const MarchBlock *__restrict__ blocks,
+                                                              const float *__restrict__ Eo, float *__restrict__ En,
+                                                              const float *__restrict__ Ho, float *__restrict__ Hn,
+                                                              const float *__restrict__ psio, float *__restrict__ psin,
+                                                              uint32_t seg, Src2 sr, FixArgs f, Deb2 db, Cap2 cp,
+                                                              uint32_t s_off) {
+    MarchBlock mb = blocks[blockIdx.x];
+    if (mb.bx == ~0u) return;
+    const uint32_t cls = mb.bz >> 30;
+    mb.bz &= 0x3FFFFFFFu;
+    const uint32_t n = f.step[0] + s_off;
+    const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
+    __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
+    if (cls == 2) march2<TJ, 2, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else if (cls == 1) march2<TJ, 1, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else march2<TJ, 0, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
 // Both steps' observers of a two-step pass (observe_pair): the first step exists only inside the
@@ -480,6 +505,16 @@ static void plan_march2(Session &s) {
     s.blk2_int.upload(cls[0].data(), cls[0].size(), s.stream);
     s.blk2_z.upload(cls[1].data(), cls[1].size(), s.stream);
     s.blk2_pml.upload(cls[2].data(), cls[2].size(), s.stream);
+    // the three lists in one for k_march2_mix, each block tagged with its class, the longest
+    // (x/y, then z) first; every list is padded to a multiple of the XCD count, so the
+    // concatenation keeps each list's XCD order
+    std::vector<MarchBlock> mix;
+    for (int c = 2; c >= 0; c--)
+        for (MarchBlock m : cls[c]) {
+            if (m.bx != ~0u) m.bz |= uint32_t(c) << 30;
+            mix.push_back(m);
+        }
+    s.blk2_mix.upload(mix.data(), mix.size(), s.stream);
 }
 
 static void plan_march(Session &s) {
@@ -597,6 +632,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves
     const char *cc = getenv("FIELDGRID_STREAMS");
     const bool side = cc ? cc[0] != '0' : nn >= (8ull << 20);
+    if (const char *m = getenv("FIELDGRID_MIX2")) s.mix2 = m[0] != '0';
     if (s.fused && side) {
         for (int q = 0; q < 2; q++) {
             s.side[q] = take_stream(dev);
@@ -1125,8 +1161,14 @@ void launch_pair(Session &s, uint32_t s_off) {
     const Src2 sr{s.src2_start.p, s.src2_ent.p};
     const Deb2 db{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
     const Cap2 cp{s.cap_start.p, s.cap_seg.p, s.cap_buf.p, s.cap_ntff.p, s.cap_plane.p};
+    // without side streams the three class launches would run one after another: one launch of
+    // the mixed list instead
+    const bool mix = !s.side[0] && s.mix2;
     hipStream_t cs[3];
-    fork_classes(s, cs);
+    if (!mix) fork_classes(s, cs);
+#define FG_MARCH2M(TJ, W, L, D)                                                                                      \
+    hipLaunchKernelGGL((k_march2_mix<TJ, W, L, D>), dim3(uint32_t(s.blk2_mix.n)), block, 0, st, g, s.blk2_mix.p, Eo, En, \
+                       Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off)
 #define FG_MARCH2X(TJ, W, L, D)                                                                                      \
     do {                                                                                                             \
         if (ni) hipLaunchKernelGGL((k_march2<TJ, 0, W, L, D>), dim3(ni), block, 0, cs[0], g, bi, Eo, En, Ho, Hn, Po, Pn, s.seg2, sr, s.fix, db, cp, s_off); \
@@ -1135,7 +1177,10 @@ void launch_pair(Session &s, uint32_t s_off) {
     } while (0)
 #define FG_MARCH2D(W, L)                                     \
     do {                                                    \
-        if (db.row || cp.start) FG_MARCH2X(12, W, L, true); \
+        if (mix) {                                          \
+            if (db.row || cp.start) FG_MARCH2M(12, W, L, true); \
+            else FG_MARCH2M(12, W, L, false);               \
+        } else if (db.row || cp.start) FG_MARCH2X(12, W, L, true); \
         else FG_MARCH2X(12, W, L, false);                   \
     } while (0)
     {
@@ -1149,7 +1194,8 @@ void launch_pair(Session &s, uint32_t s_off) {
     }
 #undef FG_MARCH2D
 #undef FG_MARCH2X
-    join_classes(s);
+#undef FG_MARCH2M
+    if (!mix) join_classes(s);
     s.cur = b;
     if (timed) FG_CHECK(hipEventRecord(timing_event(s, slot + 1), st));
     // the second step's sparse fixes: Debye from the pole states and step-n E the pass wrote

@@ -215,6 +215,8 @@ struct Session {
     bool fuse2 = false;
     uint32_t tj2 = 12, seg2 = 32;
     DevBuf<MarchBlock> blk2_int, blk2_z, blk2_pml;
+    DevBuf<MarchBlock> blk2_mix;  // the three, class-tagged, for k_march2_mix
+    bool mix2 = true;             // one mixed launch per pass where the classes would run serially
     DevBuf<uint32_t> src2_start;
     DevBuf<uint4> src2_ent;
     DevBuf<int32_t> deb2_row, deb2_map;  // Debye k maps of the two-step march (Deb2)
