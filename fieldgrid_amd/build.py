@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -41,11 +42,14 @@ def build(verbose=True):
     bat = os.path.join(tmp, "batch.o")
     adj = os.path.join(tmp, "adjoint.o")
     far = os.path.join(tmp, "far.o")
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng])
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", bat])
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "adjoint.hip"), "-o", adj])
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "far.hip"), "-o", far])
-    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sim.cpp"), "-o", sim])
+    jobs = [[hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng],
+            [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", bat],
+            [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "adjoint.hip"), "-o", adj],
+            [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "far.hip"), "-o", far],
+            ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sim.cpp"), "-o", sim]]
+    # the translation units are independent: compile them side by side
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as pool:
+        list(pool.map(_run, jobs))
     out_tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng, bat, adj, far, sim, "-o", out_tmp, "-lpthread"])
     os.replace(out_tmp, LIB)

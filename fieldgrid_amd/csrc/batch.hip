@@ -125,23 +125,29 @@ struct SpecJob {
     double dt, w;
 };
 
-__global__ __launch_bounds__(256) void k_spectra(const SpecJob *__restrict__ jobs, uint32_t n, double *__restrict__ out) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const SpecJob j = jobs[t];
-    const double re = cos(-j.w * j.dt), ri = sin(-j.w * j.dt);   // rot_e = e^{-i w dt}
-    double pr = j.dt * re, pi = j.dt * ri;                        // pe = dt e^{-i w dt}
-    double qr = j.dt * cos(-j.w * 0.5 * j.dt), qi = j.dt * sin(-j.w * 0.5 * j.dt);  // ph
+// The sums of one (port, frequency): V and I phasors into out[0..4) = {V re, V im, I re, I im}.
+__device__ __forceinline__ void spectrum(const float *__restrict__ series, uint64_t steps, uint32_t ports,
+                                         uint32_t port, double dt, double w, double *__restrict__ out) {
+    const double re = cos(-w * dt), ri = sin(-w * dt);   // rot_e = e^{-i w dt}
+    double pr = dt * re, pi = dt * ri;                    // pe = dt e^{-i w dt}
+    double qr = dt * cos(-w * 0.5 * dt), qi = dt * sin(-w * 0.5 * dt);  // ph
     double vr = 0, vi = 0, ir = 0, ii = 0;
-    for (uint64_t s = 0; s < j.steps; s++) {
-        const double v = j.series[(s * j.ports + j.port) * 2], c = j.series[(s * j.ports + j.port) * 2 + 1];
+    for (uint64_t s = 0; s < steps; s++) {
+        const double v = series[(s * ports + port) * 2], c = series[(s * ports + port) * 2 + 1];
         vr += pr * v, vi += pi * v;
         ir += qr * c, ii += qi * c;
         const double pr2 = pr * re - pi * ri, pi2 = pr * ri + pi * re;
         const double qr2 = qr * re - qi * ri, qi2 = qr * ri + qi * re;
         pr = pr2, pi = pi2, qr = qr2, qi = qi2;
     }
-    out[4 * t] = vr, out[4 * t + 1] = vi, out[4 * t + 2] = ir, out[4 * t + 3] = ii;
+    out[0] = vr, out[1] = vi, out[2] = ir, out[3] = ii;
+}
+
+__global__ __launch_bounds__(256) void k_spectra(const SpecJob *__restrict__ jobs, uint32_t n, double *__restrict__ out) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const SpecJob j = jobs[t];
+    spectrum(j.series, j.steps, j.ports, j.port, j.dt, j.w, out + 4 * size_t(t));
 }
 
 // ---- host -----------------------------------------------------------------------------------
@@ -796,3 +802,4 @@ extern "C" int fg_port_spectra(fg_device *dev, const float *series, uint64_t ste
         return int(FG_OK);
     });
 }
+

@@ -208,6 +208,14 @@ uint64_t fg_max_cells(const fg_device *dev);
 int fg_run(fg_device *dev, const fg_problem *problem, uint32_t driven, const fg_pulse *pulse,
            const fg_extras *extras, fg_progress_fn progress, void *user, fg_record *out);
 
+/* The checks fg_run makes on the host before it allocates anything, with no device needed:
+   the reference refuses a problem it cannot run in pack() and run() (run.rs:341-347: port
+   indices past f32's exact range; run.rs:401-403: more nodes than a binding holds). Here:
+   FG_INVALID for out-of-grid ids, table ranges and NULL tables, and FG_TOO_LARGE for a grid
+   whose n1*n2 planes exceed the 32-bit per-plane byte offsets of the march kernels
+   (536 870 908 nodes per plane). Capacity against free device memory is checked by fg_run. */
+int fg_problem_check(const fg_problem *problem, uint32_t driven, const fg_extras *extras);
+
 /* Thread-local text of the last error (the reference returns Result<_, String>). */
 const char *fg_last_error(void);
 

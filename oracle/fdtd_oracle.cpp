@@ -808,6 +808,26 @@ uint32_t fbits(float f) {
     return u;
 }
 
+// cpu.rs:169-193: the face weights of one sheet record (f32 in the reference for both paths)
+void sheet_weights(float *sh, float old, float ha, float hb) {
+    sh[12] = old;
+    float aa = sh[7] + ha * ha, bb = sh[8] + hb * hb, ab = sh[9] + ha * hb;
+    sh[7] = aa;
+    sh[8] = bb;
+    sh[9] = ab;
+    float sum = aa + bb;
+    if (sum <= 1e-30f) return;
+    float scale = 1.0f;
+    if (sh[6] != 0.0f) {
+        scale = sh[10] + sh[11];
+        float tot = aa + bb - 2.0f * ab;
+        if (tot > 1e-30f) scale = std::min(std::max(sum / tot, 0.5f), 4.0f);
+    }
+    if (g_mutation == 12) scale = 1.0f;  // planted: adaptive scale never applied
+    sh[10] = scale * aa / sum;
+    sh[11] = scale * bb / sum;
+}
+
 // cpu.rs:17-422
 template <class R>
 struct State {
@@ -941,25 +961,7 @@ struct State {
         return f;
     }
 
-    // cpu.rs:169-193 (sheet state is f32 in the reference for both paths)
-    void sheet_before(size_t s, float old, float ha, float hb) {
-        float *sh = &sheets[s * 16];
-        sh[12] = old;
-        float aa = sh[7] + ha * ha, bb = sh[8] + hb * hb, ab = sh[9] + ha * hb;
-        sh[7] = aa;
-        sh[8] = bb;
-        sh[9] = ab;
-        float sum = aa + bb;
-        if (sum <= 1e-30f) return;
-        float scale = 1.0f;
-        if (sh[6] != 0.0f) {
-            scale = sh[10] + sh[11];
-            float tot = aa + bb - 2.0f * ab;
-            if (tot > 1e-30f) scale = std::min(std::max(sum / tot, 0.5f), 4.0f);
-        }
-        sh[10] = scale * aa / sum;
-        sh[11] = scale * bb / sum;
-    }
+    void sheet_before(size_t s, float old, float ha, float hb) { sheet_weights(&sheets[s * 16], old, ha, hb); }
 
     // cpu.rs:195-237
     void sheet_after(size_t s) {
@@ -1213,7 +1215,9 @@ typedef struct oracle_out {
     double *e_out;
     double *h_out;
     double dt;
-    double step_seconds;  // wall time of the stepping loop alone (no build, no pack)
+    double step_seconds;  // wall time of the stepping loop from step warm_steps on (no build, no pack)
+    uint64_t warm_steps;  // in: steps stepped before the timed part of the loop starts
+    double build_seconds;  // wall time of build + pack + state set-up
 } oracle_out;
 
 }
@@ -1223,6 +1227,7 @@ typedef struct oracle_out {
 template <class R>
 static int run_impl(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pulse, const fg_extras *ex,
                     uint64_t cancel_at, double dt_override, oracle_out *out) {
+    const auto t_build = std::chrono::steady_clock::now();
     Sim<R> sim = build<R>(*desc);
     if (dt_override > 0) sim.dt = dt_override;  // the reference's pub `sim.dt` (sim.rs:176)
     out->dt = sim.dt;
@@ -1234,10 +1239,14 @@ static int run_impl(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
     double peak = 0.0, decay = 0.0;
     uint64_t calls = 0;
     int status = FG_OK;
-    const auto t_start = std::chrono::steady_clock::now();
+    auto t_start = std::chrono::steady_clock::now();
+    out->build_seconds = std::chrono::duration<double>(t_start - t_build).count();
     while (steps < ex->max_steps) {
         size_t k = std::min<size_t>(chunk, ex->max_steps - steps);
-        for (size_t s = 0; s < k; s++) st.step(steps + s);
+        for (size_t s = 0; s < k; s++) {
+            if (steps + s == out->warm_steps) t_start = std::chrono::steady_clock::now();
+            st.step(steps + s);
+        }
         steps += k;
         double e = st.energy();
         if (!std::isfinite(e)) {
@@ -1474,6 +1483,22 @@ void oracle_axis64(const double *line, uint32_t n, uint32_t pml, double dt, doub
     for (size_t i = 0; i < n; i++) {
         double r[8] = {a.d[i], a.dd[i], a.be[i], a.ce[i], a.bh[i], a.ch[i], double(a.slot_e[i]), double(a.slot_h[i])};
         std::memcpy(&rows[i * 8], r, sizeof r);
+    }
+}
+
+// The sheet face weights of cpu.rs:169-193 over a sequence of (ha, hb) face fields, for the
+// closed-form checks of the weights: a record starting with weights (ma0, mb0) and zero sums,
+// adaptive or not; writes the weights after each step.
+void oracle_sheet_weights(int adaptive, float ma0, float mb0, const float *ha, const float *hb, uint64_t n,
+                          float *ma, float *mb) {
+    float sh[16] = {0};
+    sh[6] = adaptive ? 1.0f : 0.0f;
+    sh[10] = ma0;
+    sh[11] = mb0;
+    for (uint64_t q = 0; q < n; q++) {
+        sheet_weights(sh, 0.0f, ha[q], hb[q]);
+        ma[q] = sh[10];
+        mb[q] = sh[11];
     }
 }
 

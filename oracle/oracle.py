@@ -21,7 +21,7 @@ class oracle_out(C.Structure):
                 ("series", C.POINTER(C.c_double)), ("plane", C.POINTER(C.c_double)),
                 ("ntff", C.POINTER(C.c_double)), ("probes", C.POINTER(C.c_double)),
                 ("e_out", C.POINTER(C.c_double)), ("h_out", C.POINTER(C.c_double)), ("dt", C.c_double),
-                ("step_seconds", C.c_double)]
+                ("step_seconds", C.c_double), ("warm_steps", C.c_uint64), ("build_seconds", C.c_double)]
 
 
 _lib = None
@@ -84,15 +84,17 @@ class OracleRecord:
     h: Optional[np.ndarray]
     dt: float
     step_seconds: float = 0.0
+    build_seconds: float = 0.0
 
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def run_cpu(desc, driven, pulse, extras, precision=32, cancel_at=0, dt=None, fields=True) -> OracleRecord:
+def run_cpu(desc, driven, pulse, extras, precision=32, cancel_at=0, dt=None, fields=True, warm=0) -> OracleRecord:
     """run_cpu (cpu.rs:424-482) on the oracle: precision 32 = the reference's FP32 path,
-    64 = the FP64 oracle. `desc` is a fieldgrid_amd.fdtd.SimDesc; `extras` an Extras."""
+    64 = the FP64 oracle. `desc` is a fieldgrid_amd.fdtd.SimDesc; `extras` an Extras.
+    The record's step_seconds times the stepping loop from step `warm` on."""
     n = desc.n
     nn = n[0] * n[1] * n[2]
     keep = []
@@ -106,6 +108,7 @@ def run_cpu(desc, driven, pulse, extras, precision=32, cancel_at=0, dt=None, fie
         "probes": np.zeros(max(extras.max_steps * len(extras.probes), 1)),
     }
     out = oracle_out()
+    out.warm_steps = warm
     for name, arr in bufs.items():
         setattr(out, name, _dptr(arr))
     if fields:
@@ -122,7 +125,7 @@ def run_cpu(desc, driven, pulse, extras, precision=32, cancel_at=0, dt=None, fie
     return OracleRecord(steps, out.decay_db, int(out.progress_calls), bufs["series"][: steps * ports * 2],
                         bufs["plane"][:plane_len], bufs["ntff"][: int(x.n_patches) * nf * 12],
                         bufs["probes"][: steps * len(extras.probes)], bufs.get("e"), bufs.get("h"), out.dt,
-                        out.step_seconds)
+                        out.step_seconds, out.build_seconds)
 
 
 class Packed:
