@@ -19,11 +19,14 @@
 // sets, and a planted-mistake switch (oracle_set_mutation) used only by the tests that show
 // the pinning checks catch a wrong oracle.
 //
-// Pinning: tests/test_oracle_*.py check this file against the reference's own unit tests
-// (run.rs:775-858), its analytic cavity test (tests/fdtd.rs:189-236), its cancellation and
-// wire-radius tests (tests/fdtd.rs:176-187, 271-286), and closed forms (Courant dt, CPML
-// b/c, Yee dispersion, PEC energy, CPML reflection, Debye susceptibility, resistor V/I,
-// dipole NTFF directivity and radiated power).
+// Pinning: tests/test_oracle_*.py and tests/test_antenna_dipole.py check this file against the
+// reference's own unit tests (run.rs:775-858), its analytic cavity test (tests/fdtd.rs:189-236),
+// its cancellation and wire-radius tests (tests/fdtd.rs:176-187, 271-286), its antenna bars
+// (crates/sims-antenna/tests/fdtd.rs:45-55), and closed forms and exact discrete identities
+// (Courant dt, CPML b/c, Yee dispersion, CPML reflection, Debye limits and mid-band poles
+// against Debye::response, the port's Thevenin law, lumped C and L laws, the plane DFT against
+// direct sums, the sheet impedance and face weights, dipole NTFF power balance), each shown
+// to fail on a planted mistake (oracle_set_mutation 1-14).
 
 #include <algorithm>
 #include <chrono>
@@ -387,7 +390,8 @@ Sim<R> build(const fg_sim_desc &d) {
             Edge e = to_edge(lp.edges[q]);
             size_t id = eid(e);
             if (lp.kind == FG_CAPACITOR)
-                eps_edge[e.comp][id] += lp.value * count * length(e) / area(e);
+                eps_edge[e.comp][id] += g_mutation == 10 ? lp.value * count * area(e) / length(e)  // planted: A/l for l/A
+                                                         : lp.value * count * length(e) / area(e);
             else if (lp.kind == FG_RESISTOR)
                 series_r.emplace_back(e.comp, id, lp.value / count);
         }
@@ -437,7 +441,10 @@ Sim<R> build(const fg_sim_desc &d) {
     std::vector<double> half_g(nsets);
     for (uint32_t s = 0; s < nsets; s++) {
         double acc = 0.0;
-        for (auto &[x, a] : sim.pole_sets[s]) acc += a * x * dt / (1.0 + 0.5 * x * dt);
+        for (auto [x, a] : sim.pole_sets[s]) {
+            if (g_mutation == 9) x *= 2.0;  // planted: relaxation rate doubled (tau halved)
+            acc += a * x * dt / (1.0 + 0.5 * x * dt);
+        }
         half_g[s] = acc * 0.5;
     }
     for (int c = 0; c < 3; c++)
@@ -493,7 +500,7 @@ Sim<R> build(const fg_sim_desc &d) {
             Edge e = to_edge(lp.edges[q]);
             size_t id = eid(e);
             double k_e = double(sim.cb[e.comp][id]) / area(e);
-            double k_i = dt * length(e) / (lp.value / count);
+            double k_i = dt * length(e) / (g_mutation == 11 ? lp.value * count : lp.value / count);  // planted 11: L*count
             sim.inductors.emplace_back(e.comp, id, R(k_e), R(k_i));
         }
     }
@@ -740,6 +747,7 @@ Packed<R> pack(const Sim<R> &sim, size_t driven, const fg_pulse &pulse, const fg
         for (size_t q = 0; q < p.np; q++) {
             if (q < s.size()) {
                 double x = s[q].first, a = s[q].second;
+                if (g_mutation == 9) x *= 2.0;  // planted: relaxation rate doubled (tau halved)
                 double dn = 1.0 + 0.5 * x * sim.dt;
                 p.debye_table.push_back(R((1.0 - 0.5 * x * sim.dt) / dn));
                 p.debye_table.push_back(R(EPS0 * a * x / dn));
@@ -975,6 +983,7 @@ struct State {
         size_t fa = faces + fbits(sh[14]) * stride, fb = faces + fbits(sh[15]) * stride;
         size_t b = s * (nr + 2 * nc);
         R hist = R(ma * br[fa + 1] + mb * br[fb + 1]) * R(i_old);
+        if (g_mutation == 13) hist = -hist;  // planted: sign of the i_old history term
         for (size_t k = 0; k < nr; k++) hist -= R(ma * br[fa + 2 + k] + mb * br[fb + 2 + k]) * current[b + k];
         for (size_t k = 0; k < nc; k++) {
             size_t o = 2 + nr + 2 * k;
@@ -989,7 +998,8 @@ struct State {
         sh[13] = float(R(2) * ibar - R(i_old));
         for (size_t k = 0; k < nr; k++) {
             size_t pq = 4 + 2 * k;
-            current[b + k] = R(br[pq]) * current[b + k] + R(br[pq + 1]) * ibar;
+            current[b + k] = g_mutation == 14 ? R(br[pq + 1]) * current[b + k] + R(br[pq]) * ibar  // planted: p, q swapped
+                                              : R(br[pq]) * current[b + k] + R(br[pq + 1]) * ibar;
         }
         for (size_t k = 0; k < nc; k++) {
             size_t pq = 4 + 2 * nr + 4 * k;
@@ -1095,6 +1105,7 @@ struct State {
         R dt = R(sim.dt);
         R te = (R(double(nstep)) + R(1.0)) * dt;
         R th = (R(double(nstep)) + R(0.5)) * dt;
+        if (g_mutation == 8) te = th;  // planted: plane-DFT E phase at the half step
 #pragma omp parallel for schedule(static)
         for (long qq = 0; qq < long(n[0] * n[1]); qq++) {
             size_t q = size_t(qq);

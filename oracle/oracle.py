@@ -275,3 +275,18 @@ def port_spectra(series, ports, dt, freqs):
     pe = dt * np.exp(-1j * np.outer(w, n + 1.0) * dt)
     ph = dt * np.exp(-1j * np.outer(w, n + 0.5) * dt)
     return (pe @ s[:, :, 0]).T, (ph @ s[:, :, 1]).T
+
+
+def sheet_weights(adaptive, ha, hb, start=(0.5, 0.5)):
+    """The face weights of cpu.rs:169-193 after each step of a sequence of face fields (ha, hb),
+    from a sheet record whose weights start at `start` (pack's 0.5, 0.5). Returns (ma, mb)."""
+    L = lib()
+    ha = np.ascontiguousarray(ha, np.float32)
+    hb = np.ascontiguousarray(hb, np.float32)
+    ma = np.zeros_like(ha)
+    mb = np.zeros_like(ha)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    L.oracle_sheet_weights.argtypes = [C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.oracle_sheet_weights(int(adaptive), start[0], start[1], fp(ha), fp(hb), ha.size, fp(ma), fp(mb))
+    return ma, mb

@@ -267,7 +267,9 @@ def test_planted_mistakes_are_caught(mutation):
     """Each planted mistake (oracle_set_mutation) makes its pinning check fail:
     1 psi dropped, 3 CPML b/c transposed -> CPML reflection; 2 H curl sign -> Courant;
     4 wrong axis spacing table -> Yee dispersion; 5 Debye ka for kb -> Debye limits;
-    7 conjugated NTFF E phase -> dipole power balance."""
+    7 conjugated NTFF E phase -> dipole power balance. Mutations 6 and 8-14 (source half-step,
+    plane-DFT phase, Debye relaxation time, capacitor and inductor folding, sheet branches and
+    weights) are caught in tests/test_oracle_pins.py."""
     if mutation in (1, 3):
         def check():
             try:

@@ -13,9 +13,9 @@ what the mathematics fixes:
   and its H correction terms are those closed forms;
 * the reference's own thin-wire stability test (tests/fdtd.rs:238-270) on the FP32 path.
 
-Parity unpinned: the real branch's recursion (current <- p current + q ibar, hist -= res
-current) beyond the structural identity above, and the adaptive face weights (cpu.rs:169-193);
-nothing in the reference fixes their values.
+The branches' values and the face weights are pinned in tests/test_oracle_pins.py: the sheet's
+voltage and current follow the closed-form impedance of its table row, and the weights meet
+their special cases.
 """
 import math
 
