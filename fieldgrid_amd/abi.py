@@ -177,6 +177,7 @@ SIGNATURES = {
     "fg_session_steps_per_pass": (C.c_int, [C.c_void_p, P(u32)]),
     "fg_port_spectra": (C.c_int, [C.c_void_p, P(f32), u64, u32, f64, u32, P(f64), P(f64), P(f64)]),
     "fg_problem_check": (C.c_int, [P(fg_problem), u32, P(fg_extras)]),
+    "fg_port_spectra_static": (C.c_int, [C.c_void_p, P(f32), u64, u32, f64, u32, P(f64), P(f64), P(f64)]),
     "fg_far_new": (C.c_int, [C.c_void_p, u64, P(f64), P(f64), P(f32), u32, u32, f64, f64, f64, P(C.c_void_p)]),
     "fg_far_field": (C.c_int, [C.c_void_p, u64, P(f64), P(f64)]),
     "fg_far_intensity": (C.c_int, [C.c_void_p, u64, P(f64), P(f64)]),

@@ -150,6 +150,20 @@ __global__ __launch_bounds__(256) void k_spectra(const SpecJob *__restrict__ job
     spectrum(j.series, j.steps, j.ports, j.port, j.dt, j.w, out + 4 * size_t(t));
 }
 
+// The spectra of one series with no job table: thread t = port * nf + f, written straight into
+// the caller's layout (v and i at [t * 2 + {re, im}]). w is formed as the host forms it for
+// k_spectra's jobs, so both kernels add the same terms.
+__global__ __launch_bounds__(256) void k_spectra_flat(const float *__restrict__ series, uint64_t steps, uint32_t ports,
+                                                      double dt, const double *__restrict__ freqs, uint32_t nf,
+                                                      double *__restrict__ v, double *__restrict__ i) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ports * nf) return;
+    double o[4];
+    spectrum(series, steps, ports, t / nf, dt, 2.0 * 3.14159265358979323846 * freqs[t % nf], o);
+    v[2 * size_t(t)] = o[0], v[2 * size_t(t) + 1] = o[1];
+    i[2 * size_t(t)] = o[2], i[2 * size_t(t) + 1] = o[3];
+}
+
 // ---- host -----------------------------------------------------------------------------------
 
 constexpr int N_LISTS = 12;  // CPML class (3) x wide index (2) x LDS sets (2)
@@ -803,3 +817,69 @@ extern "C" int fg_port_spectra(fg_device *dev, const float *series, uint64_t ste
     });
 }
 
+// ---- port spectra with no allocation ---------------------------------------------------------
+
+namespace fg {
+
+// The device scratch of fg_port_spectra_static: the series staged from the host, the
+// frequencies and the two output arrays, each at a 256-byte boundary. Every GPU the library
+// runs on holds its own copy until the process ends. Host code never passes `scratch` itself
+// (a host-side stand-in): it uses the address hipGetSymbolAddress gives for the device.
+constexpr uint64_t SPECTRA_SCRATCH_BYTES = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[SPECTRA_SCRATCH_BYTES];
+
+void spectra_scratch_init(fg_device *dev) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    dev->scratch = e == hipSuccess ? p : nullptr;
+    dev->scratch_error = e;
+}
+
+}  // namespace fg
+
+// fg_port_spectra without allocating: the static scratch above instead of hipMalloc'd buffers,
+// the null stream instead of a created one, and one kernel that forms each (port, frequency)
+// from the frequency table. Bit-identical to fg_port_spectra (the same sums, through
+// spectrum()).
+extern "C" int fg_port_spectra_static(fg_device *dev, const float *series, uint64_t steps, uint32_t ports, double dt,
+                                      uint32_t nf, const double *freqs, double *v, double *i) {
+    using namespace fg;
+    return guarded([&] {
+        if (!dev || (steps && ports && !series) || (nf && (!freqs || !v || !i)))
+            throw HipError{"fg_port_spectra_static: NULL argument", FG_INVALID};
+        const uint64_t n = uint64_t(ports) * nf;
+        if (!n) return int(FG_OK);
+        auto up = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+        const uint64_t cap = SPECTRA_SCRATCH_BYTES;
+        // every term is bounded before it is multiplied, so nothing below overflows
+        const bool fits = steps <= cap / 8 && n <= cap / 32 && nf <= cap / 8 &&
+                          up(steps * ports * 8) + up(uint64_t(nf) * 8) + 2 * up(n * 16) <= cap;
+        if (!fits)
+            throw HipError{"fg_port_spectra_static: " + std::to_string(steps) + " steps x " + std::to_string(ports) +
+                               " ports x " + std::to_string(nf) + " frequencies do not fit the " +
+                               std::to_string(cap >> 20) + " MiB static scratch (use fg_port_spectra)",
+                           FG_INVALID};
+        if (!dev->scratch)
+            throw HipError{std::string("fg_port_spectra_static: no static scratch on this device: ") +
+                               hipGetErrorString(dev->scratch_error),
+                           FG_HIP};
+        const uint64_t b_series = up(steps * ports * 8), b_freqs = up(uint64_t(nf) * 8), b_out = up(n * 16);
+        char *base = static_cast<char *>(dev->scratch);
+        float *ds = reinterpret_cast<float *>(base);
+        double *df = reinterpret_cast<double *>(base + b_series);
+        double *dv = reinterpret_cast<double *>(base + b_series + b_freqs);
+        double *di = reinterpret_cast<double *>(base + b_series + b_freqs + b_out);
+        std::lock_guard<std::mutex> lock(dev->scratch_mutex);
+        FG_CHECK(hipSetDevice(dev->ordinal));
+        hipStream_t st = nullptr;  // the null stream: nothing is created
+        if (steps) FG_CHECK(hipMemcpyAsync(ds, series, steps * ports * 8, hipMemcpyHostToDevice, st));
+        FG_CHECK(hipMemcpyAsync(df, freqs, uint64_t(nf) * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_spectra_flat, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, ds, steps, ports, dt, df,
+                           nf, dv, di);
+        FG_CHECK(hipGetLastError());
+        FG_CHECK(hipMemcpyAsync(v, dv, n * 16, hipMemcpyDeviceToHost, st));
+        FG_CHECK(hipMemcpyAsync(i, di, n * 16, hipMemcpyDeviceToHost, st));
+        FG_CHECK(hipStreamSynchronize(st));
+        return int(FG_OK);
+    });
+}

@@ -1340,6 +1340,7 @@ int fg_open(int ordinal, fg_device **out) {
                       double(prop.totalGlobalMem) / double(1ull << 30));
         std::unique_ptr<fg_device> dev(new fg_device{ordinal, name + info, uint64_t(prop.totalGlobalMem),
                                                      prop.multiProcessorCount});
+        fg::spectra_scratch_init(dev.get());
         *out = dev.release();
         return int(FG_OK);
     });

@@ -164,6 +164,11 @@ struct fg_device {
     // ~0.5 ms and destroying one ~0.2 ms); destroyed by fg_close
     std::vector<hipStream_t> idle_streams;
     std::mutex stream_mutex;
+    // this device's copy of the static spectra scratch (hipGetSymbolAddress at fg_open), and
+    // the lock that keeps two calls of one process from using it at once
+    void *scratch = nullptr;
+    hipError_t scratch_error = hipSuccess;
+    std::mutex scratch_mutex;
 };
 
 namespace fg {
@@ -172,6 +177,10 @@ namespace fg {
 // (synchronized) stream back.
 hipStream_t take_stream(fg_device *dev);
 void give_stream(fg_device *dev, hipStream_t s);
+
+// Looks up this device's address of the static spectra scratch (batch.hip), with the device
+// current; fg_open calls it once per device handle.
+void spectra_scratch_init(fg_device *dev);
 
 struct Session {
     fg_device *dev;

@@ -76,6 +76,17 @@ int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *const *proble
 int fg_port_spectra(fg_device *dev, const float *series, uint64_t steps, uint32_t ports, double dt, uint32_t nf,
                     const double *freqs, double *v, double *i);
 
+/* fg_port_spectra with no device allocation. It has no counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture, tight serving or
+   sweep loops, harnesses that forbid hipMalloc). Same arguments and bit-identical results; the
+   series, frequencies and outputs are staged through a static 32 MiB device scratch that each
+   GPU holds until the process ends, on the null stream. It refuses with FG_INVALID a call
+   that does not fit: 8*steps*ports + 8*nf + 32*ports*nf bytes (each rounded up to 256) above
+   32 MiB. Calls on one device share the scratch: within a process they are serialised by a
+   lock; calls from two processes on one GPU use separate copies. */
+int fg_port_spectra_static(fg_device *dev, const float *series, uint64_t steps, uint32_t ports, double dt,
+                           uint32_t nf, const double *freqs, double *v, double *i);
+
 #ifdef __cplusplus
 }
 #endif
