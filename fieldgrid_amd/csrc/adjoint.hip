@@ -542,6 +542,398 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_adj_march(Geo g, const MarchB
 #undef PSI
 }
 
+// ---- two adjoint steps per pass -----------------------------------------------------------
+//
+// k_adj_march2 runs adjoint steps n (A) and n-1 (B) in one pass over HBM, as march2 runs two
+// forward steps: it reads lam^{n+1} from one ping-pong copy and writes lam^{n-1} into the other,
+// and lam^n lives only in registers and LDS. A block is TJ+4 rows x 64 lanes (j0-2..j0+TJ+1,
+// k0-2..k0+61) owning the inner TJ x 60; step A runs on the whole tile and step B on the tile
+// less one ring, so B reads exactly the lam^n the owners of its neighbours computed. Along i
+// the block marches with a one-plane lag: iteration t runs step A at plane t and step B at plane
+// t-1. Every term is the expression k_adj_march evaluates, so two launches of k_adj_march and
+// one of k_adj_march2 give the same bits; step A's dJ/ds enters every copy of its observed
+// edges (owner or halo), and each design edge's owner adds A's gradient terms, then B's, in the
+// order two one-step launches add them. The block classes read one plane past the segment's end
+// (march_classes2 with reach 1): step A's lamH at the last plane needs lamE one plane on.
+struct AdjE {  // the inputs of one plane's hv/hu terms
+    float lx, ly, lz;
+    float2 f0, f1, f2;
+    float pxy, pxz, pyz, pyx, pzx, pzy;  // E-psi adjoints (component, axis)
+};
+
+template <int TJ, int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ + 4)) void k_adj_march2(Geo g, const MarchBlock *__restrict__ blocks,
+                                                              const float *__restrict__ Lo, float *__restrict__ Ln,
+                                                              const float *__restrict__ Mo, float *__restrict__ Mn,
+                                                              const float *__restrict__ po, float *__restrict__ pn,
+                                                              uint32_t seg, GradArgs ga, const float2 *__restrict__ tapeB,
+                                                              InjectArgs in, const double *__restrict__ dsB) {
+    constexpr int ROWS = TJ + 4;
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    // [step A, B][plane parity][row + 1][field][lane]; rows -1 and ROWS are zero pads
+    __shared__ float xv[2][2][ROWS + 2][2][64];  // hv of Ex, hu of Ez of a row (read by the row above)
+    __shared__ float xw[2][2][ROWS + 2][2][64];  // ev of Hx, eu of Hz of a row (read by the row below)
+    __shared__ float2 sets[LDS_SETS ? SETS_LDS : 1];
+    __shared__ float xa[SEG_MAX + 4][PXY ? 10 : 2];
+    const MarchBlock mb = blocks[blockIdx.x];
+    if (mb.bx == ~0u) return;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k = int(mb.bx) * OWN_K2 + lane - 2;
+    const int j = int(mb.by) * TJ + w - 2;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 2 && lane < 2 + OWN_K2 && w >= 2 && w < 2 + TJ;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    const uint32_t ob4 = off * 4u;
+    const uint32_t ldA = act ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
+    if constexpr (PML) {
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
+    }
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    // step A runs on planes tstart..i1 (its lamE on i0-1..i1 is what step B reads), step B on
+    // tstart..i1-1 and stores i0..i1-1; x-axis values of planes tstart..i1+1
+    const uint32_t tstart = i0 >= 2 ? i0 - 2 : 0;
+    for (uint32_t t = w * 64 + lane; t + tstart <= i1 + 1; t += 64 * ROWS) {
+        const uint32_t q = min(tstart + t, g.n0 - 1);
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        if constexpr (PXY) {
+            xa[t][2] = g.ax[0].bh[q], xa[t][3] = g.ax[0].ch[q], xa[t][4] = g.ax[0].raw_d[q];
+            xa[t][5] = g.ax[0].be[q], xa[t][6] = g.ax[0].ce[q], xa[t][7] = g.ax[0].raw_dd[q];
+            xa[t][8] = __int_as_float(g.ax[0].slot_h[q]), xa[t][9] = __int_as_float(g.ax[0].slot_e[q]);
+        }
+    }
+    if (w < 2)
+        for (int a = 0; a < 2; a++)
+            for (int q = 0; q < 2; q++) {
+                xv[a][q][w * (ROWS + 1)][0][lane] = 0.f, xv[a][q][w * (ROWS + 1)][1][lane] = 0.f;
+                xw[a][q][w * (ROWS + 1)][0][lane] = 0.f, xw[a][q][w * (ROWS + 1)][1][lane] = 0.f;
+            }
+    __syncthreads();  // sets and xa (slot numbers become buffer bases) are read from here on
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    // raw loads, issued one plane ahead of their use as in k_adj_march
+    struct RawE {
+        float lx, ly, lz;
+        uint32_t c0, c1, c2;
+        float pxy, pxz, pyz, pyx, pzx, pzy;
+    };
+    struct RawH {
+        float hx, hy, hz;
+        float qxy, qxz, qyz, qyx, qzx, qzy;  // H-psi adjoints
+    };
+    struct Ed {  // AdjE plus the terms it gives
+        AdjE a;
+        float hv0, hu0, hv1, hu1, hv2, hu2;
+    };
+    auto load_e = [&](uint32_t p, RawE &d) {
+        const bool in_ = p < g.n0;
+        const float *q = Lo + (in_ ? uint64_t(p) * plane : 0);
+        const uint32_t b = in_ ? pl4 : 0u;
+        d.lx = ld(q, ldA, b), d.ly = ld(q + nn, ldA, b), d.lz = ld(q + 2 * nn, ldA, b);
+        const uint64_t cb_ = in_ ? uint64_t(p) * plane : 0;
+        const uint32_t cbytes = in_ ? plc : 0u;
+        d.c0 = cidx(cb_, cbytes), d.c1 = cidx(nn + cb_, cbytes), d.c2 = cidx(2 * nn + cb_, cbytes);
+        d.pxy = d.pxz = d.pyz = d.pyx = d.pzx = d.pzy = 0.f;
+        if constexpr (PZ) {
+            const uint64_t zrow = in_ ? uint64_t(p) * g.n1 * p2 : 0;
+            const uint32_t zb = in_ ? zbytes : 0u;
+            d.pxz = ld(po + PSI(0, 0, 2) + zrow, oze, zb), d.pyz = ld(po + PSI(0, 1, 2) + zrow, oze, zb);
+        }
+        if constexpr (PXY) {
+            const float *xs = xa[min(p, i1 + 1) - tstart];
+            const int sex = __float_as_int(xs[9]);
+            const uint64_t yrow = in_ ? uint64_t(p) * p2 * g.n2 : 0, xeb = uint64_t(max(sex, 0)) * plane;
+            const uint32_t yb = in_ ? ybytes : 0u, xb = in_ ? pl4 : 0u, oxe = sex >= 0 ? ldA : OOB;
+            d.pxy = ld(po + PSI(0, 0, 1) + yrow, oye, yb);
+            d.pyx = ld(po + PSI(0, 1, 0) + xeb, oxe, xb);
+            d.pzx = ld(po + PSI(0, 2, 0) + xeb, oxe, xb), d.pzy = ld(po + PSI(0, 2, 1) + yrow, oye, yb);
+        }
+    };
+    // plane i of the adjoint H; a plane past the grid (the last segment's i1 = n0) reads zeros
+    // through empty resources
+    auto load_h = [&](uint32_t i, RawH &d) {
+        const bool in_ = i < g.n0;
+        const uint64_t pb = in_ ? uint64_t(i) * plane : 0;
+        const uint32_t hb = in_ ? pl4 : 0u;
+        d.hx = ld(Mo + pb, ldA, hb), d.hy = ld(Mo + pb + nn, ldA, hb), d.hz = ld(Mo + pb + 2 * nn, ldA, hb);
+        d.qxy = d.qxz = d.qyz = d.qyx = d.qzx = d.qzy = 0.f;
+        if constexpr (PZ) {
+            const uint64_t zrow = in_ ? uint64_t(i) * g.n1 * p2 : 0;
+            const uint32_t zb = in_ ? zbytes : 0u;
+            d.qxz = ld(po + PSI(1, 0, 2) + zrow, ozh, zb), d.qyz = ld(po + PSI(1, 1, 2) + zrow, ozh, zb);
+        }
+        if constexpr (PXY) {
+            const float *xs = xa[i - tstart];
+            const int shx = __float_as_int(xs[8]);
+            const uint64_t yrow = in_ ? uint64_t(i) * p2 * g.n2 : 0, xhb = uint64_t(max(shx, 0)) * plane;
+            const uint32_t oxh = shx >= 0 ? ldA : OOB, yb = in_ ? ybytes : 0u, xb = in_ ? pl4 : 0u;
+            d.qxy = ld(po + PSI(1, 0, 1) + yrow, oyh, yb);
+            d.qyx = ld(po + PSI(1, 1, 0) + xhb, oxh, xb);
+            d.qzx = ld(po + PSI(1, 2, 0) + xhb, oxh, xb), d.qzy = ld(po + PSI(1, 2, 1) + yrow, oyh, yb);
+        }
+    };
+    auto from_raw = [&](const RawE &r) {
+        return AdjE{r.lx, r.ly, r.lz, coef(r.c0), coef(r.c1), coef(r.c2), r.pxy, r.pxz, r.pyz, r.pyx, r.pzx, r.pzy};
+    };
+    // the hv/hu terms of plane p (k_adj_march's terms_e)
+    auto terms = [&](uint32_t p, const AdjE &a, Ed &d) {
+        d.a = a;
+        const float *xs = xa[min(p, i1 + 1) - tstart];
+        const float iddx = xs[1];
+        const float ax_ = a.f0.y != 0.f ? a.f0.y * a.lx : 0.f, ay_ = a.f1.y != 0.f ? a.f1.y * a.ly : 0.f,
+                    az_ = a.f2.y != 0.f ? a.f2.y * a.lz : 0.f;
+        d.hv0 = ax_ * iddy, d.hu0 = -ax_ * iddz;
+        d.hv1 = ay_ * iddz, d.hu1 = -ay_ * iddx;
+        d.hv2 = az_ * iddx, d.hu2 = -az_ * iddy;
+        if constexpr (PZ) {
+            d.hu0 += cez * rddz * (a.pxz - ax_);
+            d.hv1 += cez * rddz * (a.pyz + ay_);
+        }
+        if constexpr (PXY) {
+            const float cex = xs[6], rddx = xs[7];
+            d.hv0 += cey * rddy * (a.pxy + ax_);
+            d.hu1 += cex * rddx * (a.pyx - ay_);
+            d.hv2 += cex * rddx * (a.pzx + az_);
+            d.hu2 += cey * rddy * (a.pzy - az_);
+        }
+    };
+    // One transposed step at plane p on the terms of p (cur) and p+1 (nxt), the adjoint H of p
+    // (h: lamH and H-psi adjoints) and the -x carries; `pa` the step's LDS slot. Phase 1 forms
+    // lamH and its ev/eu terms; after the caller's barrier, phase 2 forms lamE and the psi
+    // adjoints of the step's output.
+    struct HOut {
+        float mx, my, mz;            // new lamH
+        float bx, by, bz;            // -k_mu lamH where the forward updates it, else 0
+        float ev0, eu0, ev1, eu1, ev2, eu2;
+        float qxy, qxz, qyz, qyx, qzx, qzy;  // new H-psi adjoints
+    };
+    auto phase1 = [&](int pa, int b, uint32_t p, const Ed &cur, const Ed &nxt, const RawH &h, HOut &o) {
+        const float *xs = xa[p - tstart];
+        const float idx_ = xs[0];
+        const bool il = p + 1 < g.n0;
+        const float x1z = lane_next(cur.hv1), x0z = lane_next(cur.hu0);
+        const float x0y = xv[pa][b][w + 2][0][lane], x2y = xv[pa][b][w + 2][1][lane];
+        o.mx = h.hx + ((cur.hv1 - x1z) + (cur.hu2 - x2y));
+        o.my = h.hy + ((cur.hv2 - nxt.hv2) + (cur.hu0 - x0z));
+        o.mz = h.hz + ((cur.hv0 - x0y) + (cur.hu1 - nxt.hu1));
+        o.bx = act && jl && kl ? -g.k_mu * o.mx : 0.f, o.by = act && kl && il ? -g.k_mu * o.my : 0.f,
+        o.bz = act && il && jl ? -g.k_mu * o.mz : 0.f;
+        o.ev0 = o.bx * idy, o.eu0 = -o.bx * idz, o.ev1 = o.by * idz, o.eu1 = -o.by * idx_, o.ev2 = o.bz * idx_,
+        o.eu2 = -o.bz * idy;
+        float bhx = 0, chx = 0, rdx = 0;
+        if constexpr (PXY) bhx = xs[2], chx = xs[3], rdx = xs[4];
+        if constexpr (PZ) {
+            o.eu0 += chz * rdz * (h.qxz - o.bx);
+            o.ev1 += chz * rdz * (h.qyz + o.by);
+        }
+        if constexpr (PXY) {
+            o.ev0 += chy * rdy * (h.qxy + o.bx);
+            o.eu1 += chx * rdx * (h.qyx - o.by);
+            o.ev2 += chx * rdx * (h.qzx + o.bz);
+            o.eu2 += chy * rdy * (h.qzy - o.bz);
+        }
+        o.qxy = h.qxy, o.qxz = h.qxz, o.qyz = h.qyz, o.qyx = h.qyx, o.qzx = h.qzx, o.qzy = h.qzy;
+        if constexpr (PZ) {
+            o.qxz = jl && kl ? bhz * (h.qxz - o.bx) : h.qxz;
+            o.qyz = kl && il ? bhz * (h.qyz + o.by) : h.qyz;
+        }
+        if constexpr (PXY) {
+            o.qxy = jl && kl ? bhy * (h.qxy + o.bx) : h.qxy;
+            o.qyx = kl && il ? bhx * (h.qyx - o.by) : h.qyx;
+            o.qzx = il && jl ? bhx * (h.qzx + o.bz) : h.qzx;
+            o.qzy = il && jl ? bhy * (h.qzy - o.bz) : h.qzy;
+        }
+        xw[pa][b][w + 1][0][lane] = o.ev0, xw[pa][b][w + 1][1][lane] = o.eu2;
+    };
+    // phase 2: lamE of plane p (before dJ/ds) into out.lx..lz, the E-psi adjoints into out.p*
+    auto phase2 = [&](int pa, int b, uint32_t p, const Ed &cur, const HOut &o, float evz_p, float euy_p, AdjE &out) {
+        const float y1z = lane_prev(o.ev1), y0z = lane_prev(o.eu0);
+        const float y0y = xw[pa][b][w][0][lane], y2y = xw[pa][b][w][1][lane];
+        const AdjE &a = cur.a;
+        float le0 = a.f0.y != 0.f ? a.lx * a.f0.x : a.lx;
+        float le1 = a.f1.y != 0.f ? a.ly * a.f1.x : a.ly;
+        float le2 = a.f2.y != 0.f ? a.lz * a.f2.x : a.lz;
+        le0 += (y1z - o.ev1) + (y2y - o.eu2);
+        le1 += (evz_p - o.ev2) + (y0z - o.eu0);
+        le2 += (y0y - o.ev0) + (euy_p - o.eu1);
+        out.lx = le0, out.ly = le1, out.lz = le2;
+        out.f0 = a.f0, out.f1 = a.f1, out.f2 = a.f2;
+        out.pxy = a.pxy, out.pxz = a.pxz, out.pyz = a.pyz, out.pyx = a.pyx, out.pzx = a.pzx, out.pzy = a.pzy;
+        if constexpr (PZ) {
+            const float ax_ = a.f0.y * a.lx, ay_ = a.f1.y * a.ly;
+            out.pxz = a.f0.y != 0.f ? bez * (a.pxz - ax_) : a.pxz;
+            out.pyz = a.f1.y != 0.f ? bez * (a.pyz + ay_) : a.pyz;
+        }
+        if constexpr (PXY) {
+            const float *xs = xa[p - tstart];
+            const float bex = xs[5];
+            const float ax_ = a.f0.y * a.lx, ay_ = a.f1.y * a.ly, az_ = a.f2.y * a.lz;
+            out.pxy = a.f0.y != 0.f ? bey * (a.pxy + ax_) : a.pxy;
+            out.pyx = a.f1.y != 0.f ? bex * (a.pyx - ay_) : a.pyx;
+            out.pzx = a.f2.y != 0.f ? bex * (a.pzx + az_) : a.pzx;
+            out.pzy = a.f2.y != 0.f ? bey * (a.pzy - az_) : a.pzy;
+        }
+    };
+    // dJ/ds into lamE at node id of plane p
+    auto inject = [&](const double *ds, uint64_t id, AdjE &out) {
+        if (!ds) return;
+        for (uint32_t q = 0; q < in.n_obs; q++) {
+            const uint64_t f = in.obs[q];
+            if (f == id) out.lx += float(ds[q]);
+            if (f == nn + id) out.ly += float(ds[q]);
+            if (f == 2 * nn + id) out.lz += float(ds[q]);
+        }
+    };
+    // the design edges' gradient terms at node (p, uj, uk), with the lamE the step started from
+    auto grad = [&](uint32_t p, const AdjE &a, const float2 *__restrict__ tape) {
+        const uint32_t bi = p - ga.lo[0], bj = uj - ga.lo[1], bk = uk - ga.lo[2];
+        if (bi < ga.nb[0] && bj < ga.nb[1] && bk < ga.nb[2]) {
+            const uint64_t bq = (uint64_t(bi) * ga.nb[1] + bj) * ga.nb[2] + bk,
+                           nb = uint64_t(ga.nb[0]) * ga.nb[1] * ga.nb[2];
+            const float lv[3] = {a.lx, a.ly, a.lz};
+            for (int c = 0; c < 3; c++) {
+                const int32_t e = ga.map[c * nb + bq];
+                if (e < 0) continue;
+                const double l = lv[c];
+                const float2 tp = tape[e];
+                ga.gca[e] += l * double(tp.x);
+                ga.gcb[e] += l * double(tp.y);
+            }
+        }
+    };
+    Ed curA, nxtA, curB, newB;
+    RawE re;
+    RawH rh;
+    load_e(tstart, re);
+    terms(tstart, from_raw(re), curA);
+    load_e(tstart + 1, re);  // tstart < i1: the block has at least one plane
+    load_h(tstart, rh);
+    xv[0][tstart & 1][w + 1][0][lane] = curA.hv0, xv[0][tstart & 1][w + 1][1][lane] = curA.hu2;
+    __syncthreads();
+    float evz_a = 0.f, euy_a = 0.f, evz_b = 0.f, euy_b = 0.f;  // the -x terms (previous plane) of A and B
+    RawH hB{};  // step A's lamH and H-psi adjoints of the previous plane: step B's input there
+    curB.a = AdjE{};
+    curB.hv0 = curB.hu0 = curB.hv1 = curB.hu1 = curB.hv2 = curB.hu2 = 0.f;
+    for (uint32_t t = tstart; t <= i1; t++) {
+        const int b = t & 1;
+        const RawE ne = re;  // plane t+1, issued the iteration before
+        const RawH h = rh;   // plane t
+        terms(t + 1, from_raw(ne), nxtA);
+        if (t < i1) {  // uniform
+            load_e(t + 2, re);
+            load_h(t + 1, rh);
+        }
+        // ---- step A at plane t, phase 1
+        HOut oa;
+        phase1(0, b, t, curA, nxtA, h, oa);
+        xv[0][b ^ 1][w + 1][0][lane] = nxtA.hv0, xv[0][b ^ 1][w + 1][1][lane] = nxtA.hu2;
+        __syncthreads();
+        // ---- step A at plane t, phase 2: lam^n of plane t, every copy
+        AdjE la;
+        phase2(0, b, t, curA, oa, evz_a, euy_a, la);
+        if (t >= g.n0) la = AdjE{};  // no plane there: its terms are the zeros k_adj_march loads
+        if (act && t < g.n0) inject(in.ds, uint64_t(t) * plane + off, la);
+        if (own && t >= i0 && t < i1) grad(t, curA.a, ga.tape);
+        evz_a = oa.ev2, euy_a = oa.eu1;
+        terms(t, la, newB);
+        // ---- step B at plane s = t-1, phase 1
+        const uint32_t s = t - 1;
+        const int sb = s & 1;
+        const bool two = t > tstart;  // uniform
+        HOut ob;
+        if (two) {
+            phase1(1, sb, s, curB, newB, hB, ob);
+            const bool wr = own && s >= i0;
+            const uint32_t sto = wr ? stO : OOB;
+            float *q = Mn + uint64_t(s) * plane;
+            st(q, sto, pl4, ob.mx), st(q + nn, sto, pl4, ob.my), st(q + 2 * nn, sto, pl4, ob.mz);
+            if constexpr (PZ) {
+                const uint64_t zrow = uint64_t(s) * g.n1 * p2;
+                const uint32_t zh = sto == OOB ? OOB : ozh;
+                st(pn + PSI(1, 0, 2) + zrow, zh, zbytes, ob.qxz), st(pn + PSI(1, 1, 2) + zrow, zh, zbytes, ob.qyz);
+            }
+            if constexpr (PXY) {
+                const float *xs = xa[s - tstart];
+                const int shx = __float_as_int(xs[8]);
+                const uint64_t yrow = uint64_t(s) * p2 * g.n2, xhb = uint64_t(max(shx, 0)) * plane;
+                const uint32_t yh = sto == OOB ? OOB : oyh, xh_ = sto == OOB || shx < 0 ? OOB : ldA;
+                st(pn + PSI(1, 0, 1) + yrow, yh, ybytes, ob.qxy);
+                st(pn + PSI(1, 1, 0) + xhb, xh_, pl4, ob.qyx);
+                st(pn + PSI(1, 2, 0) + xhb, xh_, pl4, ob.qzx);
+                st(pn + PSI(1, 2, 1) + yrow, yh, ybytes, ob.qzy);
+            }
+        }
+        xv[1][b][w + 1][0][lane] = newB.hv0, xv[1][b][w + 1][1][lane] = newB.hu2;
+        __syncthreads();
+        // ---- step B at plane s, phase 2: lam^{n-1} of plane s, stored by its owner
+        if (two) {
+            AdjE lb;
+            phase2(1, sb, s, curB, ob, evz_b, euy_b, lb);
+            const bool wr = own && s >= i0;
+            if (wr) {
+                inject(dsB, uint64_t(s) * plane + off, lb);
+                grad(s, curB.a, tapeB);
+            }
+            const uint32_t sto = wr ? stO : OOB;
+            float *q = Ln + uint64_t(s) * plane;
+            st(q, sto, pl4, lb.lx), st(q + nn, sto, pl4, lb.ly), st(q + 2 * nn, sto, pl4, lb.lz);
+            if constexpr (PZ) {
+                const uint64_t zrow = uint64_t(s) * g.n1 * p2;
+                const uint32_t ze = sto == OOB ? OOB : oze;
+                st(pn + PSI(0, 0, 2) + zrow, ze, zbytes, lb.pxz), st(pn + PSI(0, 1, 2) + zrow, ze, zbytes, lb.pyz);
+            }
+            if constexpr (PXY) {
+                const float *xs = xa[s - tstart];
+                const int sex = __float_as_int(xs[9]);
+                const uint64_t yrow = uint64_t(s) * p2 * g.n2, xeb = uint64_t(max(sex, 0)) * plane;
+                const uint32_t ye = sto == OOB ? OOB : oye, xe_ = sto == OOB || sex < 0 ? OOB : ldA;
+                st(pn + PSI(0, 0, 1) + yrow, ye, ybytes, lb.pxy);
+                st(pn + PSI(0, 1, 0) + xeb, xe_, pl4, lb.pyx);
+                st(pn + PSI(0, 2, 0) + xeb, xe_, pl4, lb.pzx);
+                st(pn + PSI(0, 2, 1) + yrow, ye, ybytes, lb.pzy);
+            }
+            evz_b = ob.ev2, euy_b = ob.eu1;
+        }
+        // rotate: plane t's step-A results become step B's input at the next iteration
+        hB.hx = oa.mx, hB.hy = oa.my, hB.hz = oa.mz;
+        hB.qxy = oa.qxy, hB.qxz = oa.qxz, hB.qyz = oa.qyz, hB.qyx = oa.qyx, hB.qzx = oa.qzx, hB.qzy = oa.qzy;
+        curA = nxtA;
+        curB = newB;
+    }
+#undef PSI
+}
+
 // dJ/ds_n enters lamE at the observed edges (s_n is E after step n)
 __global__ void k_adj_inject(Geo g, float *__restrict__ lamE, float *__restrict__ la,
                              const uint64_t *__restrict__ obs, uint32_t n_obs, const double *__restrict__ ds) {
@@ -742,6 +1134,10 @@ struct fg_adjoint {
     uint64_t n_snaps = 0, snap_len = 0;
     double ms[4] = {0, 0, 0, 0};
     hipStream_t astream = nullptr;  // the adjoint sweeps' stream (the session stream recomputes)
+    // block lists of the two-step adjoint pass (k_adj_march2), per CPML class, for segments of
+    // adj_seg planes
+    fg::DevBuf<fg::MarchBlock> ablk2[3];
+    uint32_t adj_seg = 0;
 
     ~fg_adjoint() {
         if (astream) (void)hipStreamSynchronize(astream), (void)hipStreamDestroy(astream);
@@ -865,6 +1261,51 @@ static void adj_march(Session &s, hipStream_t st, const float *Lo, float *Ln, co
 #undef FG_AM2
 }
 
+// Two adjoint steps through k_adj_march2 over the two-step adjoint block lists (march2's tile,
+// reading one plane further along i), one launch per CPML class on the stream.
+static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln, const float *Mo, float *Mn,
+                       const float *po, float *pn, const GradArgs &ga, const float2 *tapeB, const InjectArgs &in,
+                       const double *dsB) {
+    Session &s = *A.s;
+    const Geo &g = s.geo;
+    if (!A.adj_seg) {
+        const char *e = getenv("FIELDGRID_ADJ_SEG2");
+        const uint32_t sg = e ? uint32_t(std::max(1, std::min(atoi(e), SEG_MAX))) : s.seg2;
+        std::vector<MarchBlock> cls[3];
+        march_classes2(s, 12, sg, cls, 1);
+        for (int c = 0; c < 3; c++) {
+            xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
+            if (cls[c].empty()) A.ablk2[c].alloc(1);
+            else A.ablk2[c].upload(cls[c].data(), cls[c].size(), s.stream);
+            A.ablk2[c].n = cls[c].size();
+        }
+        FG_CHECK(hipStreamSynchronize(s.stream));
+        A.adj_seg = sg;
+    }
+    const dim3 block(64, 12 + 4);
+    const uint32_t n0 = uint32_t(A.ablk2[0].n), n1 = uint32_t(A.ablk2[1].n), n2 = uint32_t(A.ablk2[2].n);
+    const MarchBlock *b0 = A.ablk2[0].p, *b1 = A.ablk2[1].p, *b2 = A.ablk2[2].p;
+    const uint32_t sg = A.adj_seg;
+#define FG_AM2(W, L)                                                                                             \
+    do {                                                                                                         \
+        if (n0) hipLaunchKernelGGL((k_adj_march2<12, 0, W, L>), dim3(n0), block, 0, st, g, b0, Lo, Ln, Mo, Mn, po, pn, \
+                                   sg, ga, tapeB, in, dsB);                                                      \
+        if (n1) hipLaunchKernelGGL((k_adj_march2<12, 1, W, L>), dim3(n1), block, 0, st, g, b1, Lo, Ln, Mo, Mn, po, pn, \
+                                   sg, ga, tapeB, in, dsB);                                                      \
+        if (n2) hipLaunchKernelGGL((k_adj_march2<12, 2, W, L>), dim3(n2), block, 0, st, g, b2, Lo, Ln, Mo, Mn, po, pn, \
+                                   sg, ga, tapeB, in, dsB);                                                      \
+    } while (0)
+    const bool lds = g.n_sets <= uint32_t(SETS_LDS);
+    if (g.wide) {
+        if (lds) FG_AM2(true, true);
+        else FG_AM2(true, false);
+    } else {
+        if (lds) FG_AM2(false, true);
+        else FG_AM2(false, false);
+    }
+#undef FG_AM2
+}
+
 static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     if (!A.have_forward) throw HipError{"fg_adjoint_backward needs a forward run first", FG_INVALID};
     Session &s = *A.s;
@@ -906,6 +1347,12 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     // the two gather kernels; the march ping-pongs lamE (lamE/lamA) and lamH (lamH/lamH2)
     const char *am = getenv("FIELDGRID_ADJ_MARCH");
     const bool march = s.fused && !(am && am[0] == '0');
+    // two adjoint steps per pass (k_adj_march2) only where FIELDGRID_ADJ_PAIRS=1 asks: on the
+    // 192x192x112 design grid it halves the passes but costs 235 us per step against 150 us for
+    // one step per pass (the step is issue-bound, not bandwidth-bound: the two-step tile computes
+    // 1.15x more per owned node-step, and its x/y-slab class spills 45 VGPRs)
+    const char *ap = getenv("FIELDGRID_ADJ_PAIRS");
+    const bool pairs = march && ap && ap[0] == '1';
     float *LE[2] = {A.lamE.p, A.lamA.p}, *LH[2] = {A.lamH.p, A.lamH2.p};
     int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
@@ -979,6 +1426,15 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         for (uint64_t n = end; n-- > start;) {
             ga.tape = tape + (n - start) * ne;
             in.ds = n ? A.dseries.p + (n - 1) * n_obs : nullptr;
+            if (pairs && n >= start + 1) {  // steps n and n-1 in one pass
+                const uint64_t m = n - 1;
+                adj_march2(A, as, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga,
+                           tape + (m - start) * ne, in, m ? A.dseries.p + (m - 1) * n_obs : nullptr);
+                le ^= 1;
+                pin ^= 1;
+                n = m;  // the loop's decrement then moves past step n-1
+                continue;
+            }
             if (march) {
                 adj_march(s, as, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga, in);
                 le ^= 1;

@@ -478,7 +478,7 @@ void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<Marc
             }
 }
 
-void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]) {
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach) {
     const uint32_t nmax = s.nmax;
     std::vector<uint32_t> pre[3];
     for (int a = 0; a < 3; a++) {
@@ -502,7 +502,7 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
                 const int64_t k0 = int64_t(bx) * OWN_K2, j0 = int64_t(by) * tj, i0 = int64_t(bz) * seg;
                 const int64_t i1 = std::min<int64_t>(i0 + seg, s.n[0]);
                 const bool z = touches(2, k0 - 2, k0 + OWN_K2 + 1);
-                const bool xy = touches(1, j0 - 2, j0 + tj + 1) || touches(0, i0 - 2, i1);
+                const bool xy = touches(1, j0 - 2, j0 + tj + 1) || touches(0, i0 - 2, i1 + int64_t(reach));
                 out[xy ? 2 : z ? 1 : 0].push_back(MarchBlock{bx, by, bz});
             }
 }

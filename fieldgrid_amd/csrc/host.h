@@ -291,7 +291,9 @@ void reset(Session &s);
 // every slab, out[1] touching only z (k) slabs, out[2] touching an x or y slab.
 void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]);
 // The same for the two-step march (60 lanes x tj rows owned, two-cell halo, two-plane lag).
-void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3]);
+// The two-step march's block classes; `reach` more planes past a segment's end are read (the
+// two-step adjoint reads one: its first step's lamH at the last plane needs lamE one plane on).
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach = 0);
 
 // Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute a block
 // list so that XCD x receives a contiguous run of the logical order (k tiles fastest, then j

@@ -139,6 +139,7 @@ def _backward_both_ways(gpu, args, monkeypatch, every=40):
     d = make(gpu, args, every=every)
     s = d.forward()
     _, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq_of(args))
+    monkeypatch.setenv("FIELDGRID_ADJ_PAIRS", "0")  # one adjoint step per march pass
     monkeypatch.setenv("FIELDGRID_ADJ_MARCH", "1")
     g_march = d.backward(dJ)
     monkeypatch.setenv("FIELDGRID_ADJ_MARCH", "0")
@@ -179,3 +180,51 @@ def test_the_fused_adjoint_march_equals_the_gather_kernels_on_the_design_grid(gp
     g_march, g_gather = _backward_both_ways(gpu, args, monkeypatch, every=100)
     assert np.max(np.abs(g_gather)) > 0
     assert np.array_equal(g_march, g_gather)
+
+
+def _grids_of_every_class():
+    big, _ = W.design(dims=(40, 64, 140), dx=1e-3, pml=5, box=((12, 20), (20, 30), (60, 80)), port=(9, 30),
+                      probe=(30, 40), steps=130, sigma=0.02)
+    wide, _ = W.design(dims=(48, 64, 140), dx=1e-3, pml=5, box=((8, 38), (12, 52), (40, 100)), port=(6, 30),
+                       probe=(42, 40), steps=130, sigma=0.02)
+    small, _ = W.design_small(steps=130)
+    return [("40x64x140", big), ("48x64x140 u32 sets", wide), ("30x28x26", small)]
+
+
+def _pairs_and_singles(gpu, args, monkeypatch, every):
+    d = make(gpu, args, every=every)
+    try:
+        s = d.forward()
+        _, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq_of(args))
+        monkeypatch.setenv("FIELDGRID_ADJ_PAIRS", "1")
+        g2 = d.backward(dJ)
+        monkeypatch.setenv("FIELDGRID_ADJ_PAIRS", "0")
+        g1 = d.backward(dJ)
+        return g2, g1
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("seg", ["16", "5", "2", "1"])
+def test_two_adjoint_steps_per_pass_equal_one(gpu, monkeypatch, seg):
+    """k_adj_march2 (adjoint steps n and n-1 in one pass, lam^n in registers and LDS) against
+    k_adj_march (one step per pass), bit for bit: two-step segments of 16 down to 1 plane (warm-up
+    planes clamped at plane 0, the last segment ending at the grid's last plane), snapshot
+    intervals of 40 and 7 steps (odd segments leave one single step at each segment's start), on
+    grids with interior, z-slab and x/y-slab blocks, u16 and u32 coefficient-set indices."""
+    monkeypatch.setenv("FIELDGRID_ADJ_SEG2", seg)
+    for name, args in _grids_of_every_class():
+        for every in (40, 7):
+            print(f"two-step adjoint, {name}, planes per segment {seg}, snapshots every {every}", flush=True)
+            g2, g1 = _pairs_and_singles(gpu, args, monkeypatch, every)
+            assert np.max(np.abs(g1)) > 0
+            assert np.array_equal(g2, g1), (name, every)
+
+
+def test_two_adjoint_steps_per_pass_equal_one_on_the_design_grid(gpu, monkeypatch):
+    """The same on the inverse-design grid (192x192x112 nodes, CPML 12, u32 set indices) over
+    1000 steps."""
+    args, _ = W.design(steps=1000)
+    g2, g1 = _pairs_and_singles(gpu, args, monkeypatch, 100)
+    assert np.max(np.abs(g1)) > 0
+    assert np.array_equal(g2, g1)
