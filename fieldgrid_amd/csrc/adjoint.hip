@@ -1095,16 +1095,110 @@ __global__ __launch_bounds__(256) void k_e64(Geo64 G, double *__restrict__ E, co
 }
 
 // sources of the driven port (cpu.rs:291-297) with the pulse (sim.rs:646-657) in f64, then the
-// observations of this step
+// inductors (cpu.rs:298-303; one thread, in order: they may share an edge with a source), then
+// the observations of this step
 __global__ void k_fix64(uint64_t nn, const fg_source *__restrict__ src, uint32_t n_src, double *__restrict__ E,
                         uint64_t n, double dt, double t0, double tau, double w0, const uint64_t *__restrict__ obs,
-                        uint32_t n_obs, double *__restrict__ series) {
+                        uint32_t n_obs, double *__restrict__ series, const fg_inductor *__restrict__ ind,
+                        uint32_t n_ind, double *__restrict__ cur) {
     const uint32_t q = threadIdx.x;
     const double t = (double(n) + 0.5) * dt, x = (t - t0) / tau;
     const double pl = exp(-x * x) * cos(w0 * (t - t0));
     for (uint32_t s = q; s < n_src; s += blockDim.x) E[src[s].comp * nn + src[s].id] -= double(src[s].src) * pl;
     __syncthreads();
+    if (q == 0)
+        for (uint32_t r = 0; r < n_ind; r++) {
+            const uint64_t f = ind[r].comp * nn + ind[r].id;
+            const double v = E[f] - double(ind[r].k_e) * cur[r];
+            E[f] = v;
+            cur[r] += double(ind[r].k_i) * v;
+        }
+    __syncthreads();
     for (uint32_t o = q; o < n_obs; o += blockDim.x) series[n * n_obs + o] = E[obs[o]];
+}
+
+// The f64 forward's thin wires (cpu.rs:270-273): H[h] += sum_t c_t E[e_t] after the H update,
+// E being the step's input; each H node has one entry
+__global__ __launch_bounds__(256) void k_wires64(const fg_wire_h *__restrict__ w, uint64_t n, const double *__restrict__ E,
+                                                 double *__restrict__ H) {
+    const uint64_t q = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (q >= n) return;
+    double s = 0.0;
+    for (int t = 0; t < 4; t++) s += double(w[q].c[t]) * E[w[q].e[t]];
+    H[w[q].h] += s;
+}
+
+// The f64 forward's Debye ADE (cpu.rs:239-260): E_old saved before the E update, then after it
+// next = E - cb sum_k (1+ka_k)/2 J_k and J_k <- ka_k J_k + dd kb_k (next - E_old), with the
+// problem's pole table and the edge's cb in f64
+__global__ __launch_bounds__(256) void k_debye64(Geo64 G, FixArgs f, double *__restrict__ E, double *__restrict__ J,
+                                                 double *__restrict__ old, int save) {
+    const uint64_t d = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (d >= f.n_debye) return;
+    const fg_debye_edge e = f.debye[d];
+    if (save) {
+        old[d] = E[e.f];
+        return;
+    }
+    const float *tab = f.dtab + size_t(e.set) * f.np * 2;
+    const double cb = G.sets[G.idx[e.f]].y;
+    double s0 = 0.0;
+    for (uint32_t k = 0; k < f.np; k++) s0 += 0.5 * (1.0 + double(tab[2 * k])) * J[k * f.n_debye + d];
+    const double next = E[e.f] - cb * s0;
+    E[e.f] = next;
+    const double de = next - old[d];
+    for (uint32_t k = 0; k < f.np; k++)
+        J[k * f.n_debye + d] = double(tab[2 * k]) * J[k * f.n_debye + d] + double(e.dd) * double(tab[2 * k + 1]) * de;
+}
+
+// ---- the sparse updates of the step, transposed -------------------------------------------
+//
+// The forward step ends (after its E update) with the Debye ADE and then the inductors
+// (cpu.rs:239-260, 298-303); the adjoint step starts with their transposes, before the march:
+//   inductors, last first: x = E - k_e I; E = x; I += k_i x  gives  lamE += k_i lamI; lamI -= k_e lamE;
+//   Debye edge: next = E' - cb sum_k (1+ka_k)/2 J_k; J_k <- ka_k J_k + dd kb_k (next - E_old) gives
+//     mu = lamE + sum_k dd kb_k lamJ_k (the adjoint of E'), lamJ_k <- ka_k lamJ_k - cb (1+ka_k)/2 mu,
+//     and -sum_k dd kb_k lamJ_k (the adjoint of E_old = E^n), which k_adj_debye_post adds to the
+//     march's output once the E update's transpose has turned mu into the rest of lamE^n.
+// Inductor edges are lumped edges, which carry no Debye medium, so the two sets are disjoint.
+__global__ __launch_bounds__(256) void k_adj_media_pre(Geo g, FixArgs f, float *__restrict__ lamE,
+                                                       float *__restrict__ lamJ, float *__restrict__ lamI,
+                                                       float *__restrict__ corr) {
+    const uint64_t d = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (d == 0)
+        for (uint32_t q = f.n_ind; q-- > 0;) {
+            const uint64_t fl = f.ind[q].comp * g.nn + f.ind[q].id;
+            lamE[fl] += f.ind[q].k_i * lamI[q];
+            lamI[q] -= f.ind[q].k_e * lamE[fl];
+        }
+    if (d >= f.n_debye) return;
+    const fg_debye_edge e = f.debye[d];
+    const float *tab = f.dtab + size_t(e.set) * f.np * 2;
+    const float cb = coef_at(g, e.f).y;
+    float back = 0.f;
+    for (uint32_t k = 0; k < f.np; k++) back += e.dd * tab[2 * k + 1] * lamJ[k * f.n_debye + d];
+    const float mu = lamE[e.f] + back;
+    corr[d] = back;
+    for (uint32_t k = 0; k < f.np; k++)
+        lamJ[k * f.n_debye + d] = tab[2 * k] * lamJ[k * f.n_debye + d] - cb * (0.5f * (1.0f + tab[2 * k])) * mu;
+    lamE[e.f] = mu;
+}
+
+__global__ __launch_bounds__(256) void k_adj_debye_post(FixArgs f, float *__restrict__ lamE, const float *__restrict__ corr) {
+    const uint64_t d = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (d < f.n_debye) lamE[f.debye[d].f] -= corr[d];
+}
+
+// The thin wires' transpose (cpu.rs:270-273): H'' = H' + W E^n gives lamE^n[e_t] += c_t lamH[h],
+// lamH being the adjoint of the corrected H (the march's new lamH). One thread, since the
+// terms of several H nodes may name one E edge.
+__global__ void k_adj_wires_post(const fg_wire_h *__restrict__ w, uint64_t n, float *__restrict__ lamE,
+                                 const float *__restrict__ lamH) {
+    if (threadIdx.x || blockIdx.x) return;
+    for (uint64_t q = 0; q < n; q++) {
+        const float l = lamH[w[q].h];
+        for (int t = 0; t < 4; t++) lamE[w[q].e[t]] += w[q].c[t] * l;
+    }
 }
 
 }  // namespace fg
@@ -1138,6 +1232,10 @@ struct fg_adjoint {
     // adj_seg planes
     fg::DevBuf<fg::MarchBlock> ablk2[3];
     uint32_t adj_seg = 0;
+    // the step's sparse state (Debye pole states, inductor currents): its length in the
+    // snapshots, and its adjoint (lamJ, lamI) with the Debye edges' E_old terms (corr)
+    uint64_t dlen = 0, ilen = 0;
+    fg::DevBuf<float> lamJ, lamI, corr;
 
     ~fg_adjoint() {
         if (astream) (void)hipStreamSynchronize(astream), (void)hipStreamDestroy(astream);
@@ -1160,9 +1258,11 @@ static void rebuild(fg_adjoint &A) {
     if (fg_sim_new(&A.desc, &A.sim) != FG_OK) throw HipError{g_error, FG_INVALID};
     uint64_t counts[4];
     fg_sim_counts(A.sim, counts);
-    if (counts[0] || counts[1] || counts[2] || counts[3])
-        throw HipError{"the adjoint covers curl updates, CPML and resistive ports; this problem has Debye media, "
-                       "inductors, resistive sheets or thin wires",
+    // resistive sheets weigh their faces by the H^2 the run has accumulated (cpu.rs:169-193), so a
+    // step with sheets is not linear in the fields and has no transpose to run backward
+    if (counts[2])
+        throw HipError{"the adjoint covers curl updates, CPML, resistive ports, Debye media, inductors and thin "
+                       "wires; resistive sheets make the step nonlinear in the fields",
                        FG_UNSUPPORTED};
     if (fg_sim_pack(A.sim, 0, &A.problem) != FG_OK) throw HipError{g_error, FG_INVALID};
     fg_extras x{};
@@ -1173,6 +1273,8 @@ static void rebuild(fg_adjoint &A) {
     x.n_probes = uint32_t(A.obs.size());
     x.probes = A.obs.data();
     A.s.reset(session_new(A.dev, &A.problem, A.driven, &A.pulse, &x));
+    A.dlen = A.s->fix.n_debye ? uint64_t(A.s->fix.np) * A.s->fix.n_debye : 0;
+    A.ilen = A.s->fix.n_ind;
     if (A.edges.empty()) {
         const std::string err = design_map(*A.sim, A.desc, A.design.lo, A.design.hi, A.edges);
         if (!err.empty()) throw HipError{err, FG_INVALID};
@@ -1204,7 +1306,7 @@ static void forward(fg_adjoint &A, float *series) {
     reset(s);
     s.observe = true;
     const uint64_t psi_n = s.psi[0].n, nn = s.nn;
-    A.snap_len = 6 * nn + psi_n;
+    A.snap_len = 6 * nn + psi_n + A.dlen + A.ilen;
     A.n_snaps = (A.steps + A.every - 1) / A.every;  // segment k starts at k*every; k = 0 is all zeros
     if (A.snaps.n < (A.n_snaps ? A.n_snaps - 1 : 0) * A.snap_len) {
         size_t free_b = 0, total_b = 0;
@@ -1221,6 +1323,9 @@ static void forward(fg_adjoint &A, float *series) {
             FG_CHECK(hipMemcpyAsync(d, s.E[s.cur].p, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
             FG_CHECK(hipMemcpyAsync(d + 3 * nn, s.H[s.cur].p, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
             if (psi_n) FG_CHECK(hipMemcpyAsync(d + 6 * nn, s.psi[s.cur].p, psi_n * 4, hipMemcpyDeviceToDevice, st));
+            if (A.dlen) FG_CHECK(hipMemcpyAsync(d + 6 * nn + psi_n, s.dstate.p, A.dlen * 4, hipMemcpyDeviceToDevice, st));
+            if (A.ilen)
+                FG_CHECK(hipMemcpyAsync(d + 6 * nn + psi_n + A.dlen, s.ind_i.p, A.ilen * 4, hipMemcpyDeviceToDevice, st));
         }
         enqueue_steps(s, std::min<uint64_t>(A.every, A.steps - k * A.every));
     }
@@ -1342,17 +1447,24 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         }
         A.dmap.upload(map.data(), map.size(), st);
     }
+    const bool media = A.dlen || A.ilen || s.wires.n;
+    if (media) {
+        if (A.lamJ.n != std::max<uint64_t>(A.dlen, 1)) A.lamJ.alloc(std::max<uint64_t>(A.dlen, 1));
+        if (A.lamI.n != std::max<uint64_t>(A.ilen, 1)) A.lamI.alloc(std::max<uint64_t>(A.ilen, 1));
+        if (A.corr.n != std::max<uint64_t>(s.fix.n_debye, 1)) A.corr.alloc(std::max<uint64_t>(s.fix.n_debye, 1));
+        A.lamJ.zero(st), A.lamI.zero(st);
+    }
     A.lamE.zero(st), A.lamA.zero(st), A.lamH.zero(st), A.lamH2.zero(st), A.lamP[0].zero(st), A.lamP[1].zero(st);
     // the fused adjoint march (one pass per adjoint step) unless FIELDGRID_ADJ_MARCH=0 asks for
     // the two gather kernels; the march ping-pongs lamE (lamE/lamA) and lamH (lamH/lamH2)
     const char *am = getenv("FIELDGRID_ADJ_MARCH");
-    const bool march = s.fused && !(am && am[0] == '0');
+    const bool march = s.fused && (!(am && am[0] == '0') || media);  // the gather kernels have no media
     // two adjoint steps per pass (k_adj_march2) only where FIELDGRID_ADJ_PAIRS=1 asks: on the
     // 192x192x112 design grid it halves the passes but costs 235 us per step against 150 us for
     // one step per pass (the step is issue-bound, not bandwidth-bound: the two-step tile computes
     // 1.15x more per owned node-step, and its x/y-slab class spills 45 VGPRs)
     const char *ap = getenv("FIELDGRID_ADJ_PAIRS");
-    const bool pairs = march && ap && ap[0] == '1';
+    const bool pairs = march && !media && ap && ap[0] == '1';
     float *LE[2] = {A.lamE.p, A.lamA.p}, *LH[2] = {A.lamH.p, A.lamH2.p};
     int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
@@ -1391,11 +1503,16 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         // restore the segment's starting fields into copy 0
         if (k == 0) {
             s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
+            if (A.dlen) s.dstate.zero(st);
+            if (A.ilen) FG_CHECK(hipMemcpyAsync(s.ind_i.p, s.ind_i0.data(), A.ilen * 4, hipMemcpyHostToDevice, st));
         } else {
             const float *src = A.snaps.p + (k - 1) * A.snap_len;
             FG_CHECK(hipMemcpyAsync(s.E[0].p, src, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
             FG_CHECK(hipMemcpyAsync(s.H[0].p, src + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
             if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, src + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
+            if (A.dlen) FG_CHECK(hipMemcpyAsync(s.dstate.p, src + 6 * nn + psi_n, A.dlen * 4, hipMemcpyDeviceToDevice, st));
+            if (A.ilen)
+                FG_CHECK(hipMemcpyAsync(s.ind_i.p, src + 6 * nn + psi_n + A.dlen, A.ilen * 4, hipMemcpyDeviceToDevice, st));
         }
         s.cur = 0;
         s.steps = start;
@@ -1436,7 +1553,15 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
                 continue;
             }
             if (march) {
+                const uint32_t mb = uint32_t((std::max<uint64_t>(s.fix.n_debye, 1) + 255) / 256);
+                if (media)
+                    hipLaunchKernelGGL(k_adj_media_pre, dim3(mb), dim3(256), 0, as, g, s.fix, LE[le], A.lamJ.p, A.lamI.p,
+                                       A.corr.p);
                 adj_march(s, as, LE[le], LE[le ^ 1], LH[le], LH[le ^ 1], A.lamP[pin].p, A.lamP[pin ^ 1].p, ga, in);
+                if (s.fix.n_debye) hipLaunchKernelGGL(k_adj_debye_post, dim3(mb), dim3(256), 0, as, s.fix, LE[le ^ 1], A.corr.p);
+                if (s.wires.n)
+                    hipLaunchKernelGGL(k_adj_wires_post, dim3(1), dim3(64), 0, as, s.wires.p, uint64_t(s.wires.n), LE[le ^ 1],
+                                       LH[le ^ 1]);
                 le ^= 1;
             } else {
                 hipLaunchKernelGGL(k_adj_e, grid, block, 0, as, g, A.lamA.p, A.lamE.p, A.lamH.p, A.lamP[pin].p, ga);
@@ -1539,11 +1664,22 @@ static void forward64(fg_adjoint &A, const double *eps_box, double *series) {
     const double tau = std::sqrt(std::log(10.0)) / (3.14159265358979323846 * A.pulse.fc), t0 = 4.0 * tau;
     const double w0 = 2.0 * 3.14159265358979323846 * A.pulse.f0;
     const dim3 block(64, 4), grid((s.n[2] + 63) / 64, (s.n[1] + 3) / 4, s.n[0]);
+    // the step's sparse state in f64: Debye pole states and E_old, inductor currents
+    const FixArgs &fx = s.fix;
+    DevBuf<double> J, old, cur;
+    J.alloc(std::max<uint64_t>(A.dlen, 1)), old.alloc(std::max<uint64_t>(fx.n_debye, 1));
+    cur.alloc(std::max<uint64_t>(A.ilen, 1));
+    J.zero(st), cur.zero(st);
+    const uint32_t db = uint32_t((std::max<uint64_t>(fx.n_debye, 1) + 255) / 256);
+    const uint32_t wb = uint32_t((std::max<uint64_t>(s.wires.n, 1) + 255) / 256);
     for (uint64_t n = 0; n < A.steps; n++) {
         hipLaunchKernelGGL(k_h64, grid, block, 0, st, G, E.p, H.p, psi.p);
+        if (s.wires.n) hipLaunchKernelGGL(k_wires64, dim3(wb), dim3(256), 0, st, s.wires.p, uint64_t(s.wires.n), E.p, H.p);
+        if (fx.n_debye) hipLaunchKernelGGL(k_debye64, dim3(db), dim3(256), 0, st, G, fx, E.p, J.p, old.p, 1);
         hipLaunchKernelGGL(k_e64, grid, block, 0, st, G, E.p, H.p, psi.p);
+        if (fx.n_debye) hipLaunchKernelGGL(k_debye64, dim3(db), dim3(256), 0, st, G, fx, E.p, J.p, old.p, 0);
         hipLaunchKernelGGL(k_fix64, dim3(1), dim3(64), 0, st, nn, src.p, n_src, E.p, n, dt, t0, tau, w0, dobs.p,
-                           uint32_t(n_obs), out.p);
+                           uint32_t(n_obs), out.p, fx.ind, fx.n_ind, cur.p);
     }
     FG_CHECK(hipGetLastError());
     if (series && A.steps * n_obs)

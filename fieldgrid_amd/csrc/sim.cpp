@@ -637,6 +637,14 @@ static void design_map_impl(const fg_sim &s, const fg_sim_desc &d, const uint32_
     }
     for (uint32_t r = 0; r < d.n_resistive; r++) excluded.insert(uint64_t(d.resistive[r].comp) * nn + d.resistive[r].id);
     for (const fg_debye_edge &e : s.debye) debye.insert(e.f);
+    // a thin wire rescales the edges and H nodes around it (sim.rs:556-635), so eps_r enters
+    // those edges through another formula: the box keeps clear of every wire by two nodes
+    for (uint32_t q = 0; q < d.n_wires; q++) {
+        const fg_edge &e = d.wires[q].edge;
+        bool near = true;
+        for (int a = 0; a < 3; a++) near = near && int64_t(e.at[a]) + 2 >= int64_t(lo[a]) && int64_t(e.at[a]) <= int64_t(hi[a]) + 1;
+        if (near) throw BuildError{"the design box touches a thin wire"};
+    }
     out.clear();
     static const int offs[4][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}};
     for (uint64_t ci = lo[0]; ci < hi[0]; ci++)

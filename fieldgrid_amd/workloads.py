@@ -201,10 +201,25 @@ def design(dims=(192, 192, 112), dx=0.5e-3, pml=12, box=((64, 128), (64, 128), (
                 steps=steps), freq
 
 
-def design_small(steps=300, sigma=0.02, seed=5):
-    """The design recipe at oracle size: 30x28x26 nodes, 1 mm, CPML 5, a 6x5x4-cell box."""
-    return design(dims=(30, 28, 26), dx=1e-3, pml=5, box=((11, 17), (11, 16), (10, 14)), port=(7, 14),
-                  probe=(23, 13), steps=steps, freq=8e9, seed=seed, sigma=sigma)
+def design_small(steps=300, sigma=0.02, seed=5, media=False):
+    """The design recipe at oracle size: 30x28x26 nodes, 1 mm, CPML 5, a 6x5x4-cell box. With
+    `media`, the wave from the port also crosses a block of Debye medium between the box and the
+    observation (eps_inf 2, delta 1.5, tau 20 ps), a 3 nH lumped inductor over two z edges and a
+    thin z wire of 0.1 mm radius over four edges, all outside the box."""
+    args, freq = design(dims=(30, 28, 26), dx=1e-3, pml=5, box=((11, 17), (11, 16), (10, 14)), port=(7, 14),
+                        probe=(23, 13), steps=steps, freq=8e9, seed=seed, sigma=sigma)
+    if media:
+        grid, mats = args["grid"], args["mats"]
+        n = grid.dims()
+        cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+        sl = (slice(18, 22), slice(9, 19), slice(8, 18))
+        mats.eps.reshape(cells)[sl] = 3.5
+        mats.inf.reshape(cells)[sl] = 2.0
+        mats.debye.reshape(cells)[sl] = 1.5
+        args["media"] = F.Media(debye=F.Debye([(1.0 / 20e-12, 1.0)]),
+                                wires=[(F.Edge(2, (9, 21, k)), 0.1e-3) for k in range(11, 15)])
+        args["lumped"] = [F.Lumped("l", [F.Edge(2, (9, 8, k)) for k in (12, 13)], F.Element.Inductor(3e-9))]
+    return args, freq
 
 
 def dft_objective(series, dt, freq):

@@ -9,7 +9,9 @@
  *   backward(dseries) -> grad    dJ/d eps_r per box cell, given dJ/d series from the caller's
  *                                objective J(series)
  * The backward pass is the exact transpose of the discrete FDTD step: the curl updates, the CPML
- * psi recursions and the resistive ports (their (ca, cb)). The step is linear and time-invariant
+ * psi recursions, the resistive ports (their (ca, cb)), the Debye ADE (cpu.rs:239-260; its pole
+ * states are state), lumped inductors (cpu.rs:298-303; their currents are state) and thin-wire H
+ * corrections (cpu.rs:270-273). The step is linear and time-invariant
  * in the fields, so the adjoint recursion runs backward in time on its own fields. The forward
  * fields enter only through E and the curl term on the design edges. The forward pass keeps a
  * snapshot of the fields every `every` steps. The backward pass recomputes one segment at a time
@@ -17,9 +19,11 @@
  * second stream while the next (earlier) segment is recomputed. Memory is the snapshots plus two
  * segments' tapes.
  *
- * Supported problems: curl updates, CPML and resistive ports. Debye media, lumped inductors,
- * resistive sheets and thin wires give FG_UNSUPPORTED. The box must not touch a port, lumped or
- * resistive edge, nor reach into the CPML (FG_INVALID).
+ * Supported problems: every Sim feature but resistive sheets, whose face weights follow the H^2
+ * the run has accumulated (cpu.rs:169-193), so a step with sheets is not linear in the fields:
+ * they give FG_UNSUPPORTED. The box must not touch a port, lumped or resistive edge, come within
+ * two nodes of a thin wire (the wire rescales the edges around it), nor reach into the CPML
+ * (FG_INVALID). Snapshots hold the pole states and inductor currents with the fields.
  *
  * fg_adjoint_forward64 runs the same problem in f64 with the box's edge coefficients evaluated
  * in f64 from a given eps_r: a smooth objective for central finite differences.

@@ -1534,8 +1534,12 @@ int oracle_check(const fg_sim_desc *desc) {
 // (ca, cb) of the edges whose material average (sim.rs:380-410) reads a design cell:
 //   dJ/dca_e = sum_n lamE_e^{n+1} E_e^n,  dJ/dcb_e = sum_n lamE_e^{n+1} curl_e^n,
 // with lam^{n} = M^T lam^{n+1} + dJ/dx^n. M^T is written here by transposing each statement of
-// update_e (cpu.rs:109-167) and then of update_h (cpu.rs:48-107) in scatter form, statement for
-// statement; the HIP engine uses a gather form derived separately.
+// the step (cpu.rs:267-307) in reverse order, in scatter form, statement for statement: the
+// inductors (cpu.rs:298-303; their currents are state), the Debye ADE (cpu.rs:239-260; the pole
+// states are state, E^n enters through E_old), update_e (cpu.rs:109-167), the thin wires
+// (cpu.rs:270-273) and update_h (cpu.rs:48-107); the HIP engine uses a gather form derived
+// separately. Resistive sheets are refused: their face weights follow the accumulated H^2 of
+// the run (cpu.rs:169-193), so the step is not linear in the fields.
 namespace {
 
 struct DesignEdge {
@@ -1569,6 +1573,12 @@ std::vector<DesignEdge> design_edges(const fg_sim_desc &d, const Sim<double> &si
     for (uint32_t r = 0; r < d.n_resistive; r++) excluded.insert({d.resistive[r].comp, size_t(d.resistive[r].id)});
     std::set<std::pair<size_t, size_t>> debye;
     for (auto &[c, id, dd, set] : sim.debye_edges) debye.insert({c, id});
+    for (uint32_t q = 0; q < d.n_wires; q++) {  // the edges and H nodes around a wire are rescaled
+        const fg_edge &e = d.wires[q].edge;
+        bool near = true;
+        for (int a = 0; a < 3; a++) near = near && long(e.at[a]) + 2 >= long(lo[a]) && long(e.at[a]) <= long(hi[a]) + 1;
+        if (near) throw Err{"the design box touches a thin wire"};
+    }
     std::vector<DesignEdge> out;
     for (size_t ci = lo[0]; ci < hi[0]; ci++)
         for (size_t cj = lo[1]; cj < hi[1]; cj++)
@@ -1617,8 +1627,7 @@ int design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
                     const fg_point_probe *obs, const uint32_t lo[3], const uint32_t hi[3], const double *dseries,
                     double *grad, double *series_out) {
     Sim<double> sim = build<double>(*desc);
-    if (!sim.inductors.empty() || !sim.sheets.empty() || !sim.wires.empty() || !sim.debye_edges.empty())
-        throw Err{"the adjoint covers curl updates, CPML and resistive ports only"};
+    if (!sim.sheets.empty()) throw Err{"resistive sheets make the step nonlinear in the fields"};
     for (uint32_t q = 0; q < n_obs; q++)
         if (obs[q].field > 2) throw Err{"observations are E components"};
     fg_extras ex{};
@@ -1645,6 +1654,8 @@ int design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
         for (size_t q = 0; q < steps * n_obs; q++) series_out[q] = st.probes[q];
     std::vector<double> le(3 * nn, 0.0), lh(3 * nn, 0.0), lp(st.psi.size(), 0.0);
     std::vector<double> gca(ne, 0.0), gcb(ne, 0.0);
+    const size_t nd = sim.debye_edges.size(), np = st.p.np, nind = st.p.ind_f.size();
+    std::vector<double> lj(np * nd, 0.0), dold(nd, 0.0), li(nind, 0.0);
     const Axis<double> *ax = sim.ax;
     const size_t pml = sim.pml;
     const double k_mu = sim.dt / MU0;
@@ -1655,6 +1666,29 @@ int design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
             double l = le[de[q].c * nn + de[q].id];
             gca[q] += l * tape_e[s * ne + q];
             gcb[q] += l * tape_c[s * ne + q];
+        }
+        // transpose of the inductors, last first (cpu.rs:298-303):
+        //   x = E - k_e I; E = x; I += k_i x
+        for (size_t q = nind; q-- > 0;) {
+            const size_t f = st.p.ind_f[q];
+            const double k_e = st.inductor[q * 5 + 2], k_i = st.inductor[q * 5 + 3];
+            le[f] += k_i * li[q];
+            li[q] -= (g_mutation == 17 ? -k_e : k_e) * le[f];  // planted 17: k_e sign
+        }
+        // transpose of the Debye ADE (cpu.rs:239-260): next = E' - cb sum_k (1+ka_k)/2 J_k,
+        // J_k <- ka_k J_k + dd kb_k (next - E_old), E_old = E^n
+        for (size_t d = 0; d < nd; d++) {
+            auto &[c, id, dd, set] = sim.debye_edges[d];
+            const size_t f = c * nn + id;
+            const double *tab = &st.p.debye_table[size_t(set) * np * 2];
+            const double cb = sim.cb[c][id];
+            double mu = le[f], back = 0.0;
+            for (size_t k = 0; k < np; k++) back += dd * tab[2 * k + 1] * lj[k * nd + d];
+            if (g_mutation == 15) back = 0.0;  // planted: pole states left out of the adjoint
+            mu += back;      // adjoint of next, through E^{n+1} and the pole states
+            dold[d] = -back;  // adjoint of E_old = E^n
+            for (size_t k = 0; k < np; k++) lj[k * nd + d] = tab[2 * k] * lj[k * nd + d] - cb * 0.5 * (1.0 + tab[2 * k]) * mu;
+            le[f] = mu;  // adjoint of E' (the E update's output)
         }
         // transpose of update_e: E_c = ca E_c + cb (t1 - t2), t1 = hv inv_dd_u + psi_u', ...
         for (size_t i = 0; i < n[0]; i++)
@@ -1693,6 +1727,14 @@ int design_gradient(const fg_sim_desc *desc, uint32_t driven, const fg_pulse *pu
                         lh[u * nn + idx(n, vm[0], vm[1], vm[2])] -= hu;
                     }
                 }
+        for (size_t d = 0; d < nd; d++) {
+            auto &[c, id, dd, set] = sim.debye_edges[d];
+            le[c * nn + id] += dold[d];
+        }
+        // transpose of the thin wires (cpu.rs:270-273): H[h] += sum_t c_t E^n[e_t]
+        if (g_mutation != 16)  // planted 16: wires left out of the adjoint
+            for (auto &w : sim.wires)
+                for (int t = 0; t < 4; t++) le[w.e[t]] += w.c[t] * lh[w.h];
         // transpose of update_h: H_c -= k_mu (t1 - t2), t1 = ev inv_d_u + psi_u', ...
         for (size_t i = 0; i < n[0]; i++)
             for (size_t j = 0; j < n[1]; j++)

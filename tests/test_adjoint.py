@@ -15,6 +15,8 @@ import pytest
 from fieldgrid_amd import abi
 from fieldgrid_amd import fdtd as F
 from fieldgrid_amd import workloads as W
+
+from .fixtures import sheet_table
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -125,14 +127,17 @@ def test_problems_outside_the_adjoints_reach_are_refused(gpu):
     with pytest.raises(abi.FieldgridError) as e:
         make(gpu, port)
     assert e.value.status == abi.FG_INVALID and "port" in str(e.value)
-    debye = dict(args, media=F.Media(debye=F.Debye([(1e11, 1.0)])))
-    args["mats"].debye[:] = 0.0
-    m = F.Materials.new(args["grid"])
-    m.eps[:] = args["mats"].eps
-    m.debye[:50] = 1.0
+    # resistive sheets weigh their faces by the run's accumulated H^2: not linear, refused
+    n = args["grid"].dims()
+    sheet = dict(args, resistive=[F.Resistive(0, F.idx(n, 22, 20, 12), 30.0, False, (0, 1))],
+                 media=F.Media(sheets=sheet_table()))
     with pytest.raises(abi.FieldgridError) as e:
-        make(gpu, dict(debye, mats=m))
-    assert e.value.status == abi.FG_UNSUPPORTED
+        make(gpu, sheet)
+    assert e.value.status == abi.FG_UNSUPPORTED and "nonlinear" in str(e.value)
+    wire = dict(args, media=F.Media(wires=[(F.Edge(2, (12, 12, k)), 0.1e-3) for k in range(14, 17)]))
+    with pytest.raises(abi.FieldgridError) as e:
+        make(gpu, wire)
+    assert e.value.status == abi.FG_INVALID and "thin wire" in str(e.value)
 
 
 def _backward_both_ways(gpu, args, monkeypatch, every=40):
@@ -228,3 +233,67 @@ def test_two_adjoint_steps_per_pass_equal_one_on_the_design_grid(gpu, monkeypatc
     g2, g1 = _pairs_and_singles(gpu, args, monkeypatch, 100)
     assert np.max(np.abs(g1)) > 0
     assert np.array_equal(g2, g1)
+
+
+# ---- Debye media, inductors and thin wires in the adjoint ----------------------------------------
+
+def test_the_media_gradient_matches_the_fp64_adjoint(gpu):
+    """The design problem with a Debye block (pole states are state), a lumped inductor (its
+    current is state) and a thin wire between the port and the observation: the forward is the
+    FP32 path bit for bit, and the gradient agrees with the FP64 oracle adjoint (whose media
+    transposes are pinned by central differences in tests/test_adjoint_oracle.py)."""
+    args, freq = W.design_small(steps=300, media=True)
+    d = make(gpu, args, every=64)
+    try:
+        s = d.forward()
+        ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
+        o32 = O.run_cpu(d.desc, args["driven"], args["pulse"], ex, 32, fields=False)
+        assert np.array_equal(s[:, 0], np.asarray(o32.probes, np.float32))
+        J, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq)
+        g = d.backward(dJ)
+        go, _ = O.design_gradient(d.desc, args["driven"], args["pulse"], args["steps"], args["obs"], args["lo"],
+                                  args["hi"], dJ)
+        rel = np.max(np.abs(g - go)) / np.max(np.abs(go))
+        print(f"\nmedia: J = {J:.4e}; max |grad - fp64 adjoint| / max |fp64 adjoint| = {rel:.2e}")
+        assert rel < 1e-3
+    finally:
+        d.close()
+
+
+def test_the_media_gradient_matches_central_differences_of_the_f64_forward(gpu):
+    args, freq = W.design_small(steps=300, sigma=0.02, media=True)
+    d = make(gpu, args, every=50)
+    try:
+        s = d.forward()
+        dt = F.time_step(args["grid"])
+        _, dJ = W.dft_objective(s[:, 0], dt, freq)
+        g = d.backward(dJ)
+        box = box_of(args).astype(np.float64)
+        rng = np.random.default_rng(4)
+        h = 1e-3
+        for _ in range(5):
+            c = tuple(int(rng.integers(0, n)) for n in box.shape)
+            up, dn = box.copy(), box.copy()
+            up[c] += h
+            dn[c] -= h
+            fd = (W.dft_objective(d.forward64(up)[:, 0], dt, freq)[0] -
+                  W.dft_objective(d.forward64(dn)[:, 0], dt, freq)[0]) / (2 * h)
+            print(f"media cell {c}: adjoint {g[c]:.6e}  central difference {fd:.6e}")
+            assert abs(g[c] - fd) < 1e-3 * max(abs(fd), 1e-3 * np.max(np.abs(g)))
+    finally:
+        d.close()
+
+
+def test_the_media_state_survives_the_snapshots(gpu):
+    """The pole states and inductor currents are part of every snapshot: the gradient does not
+    depend on the snapshot interval."""
+    args, freq = W.design_small(steps=200, media=True)
+    grads = []
+    for every in (7, 200):
+        d = make(gpu, args, every=every)
+        s = d.forward()
+        _, dJ = W.dft_objective(s[:, 0], F.time_step(args["grid"]), freq)
+        grads.append(d.backward(dJ))
+        d.close()
+    assert np.max(np.abs(grads[0])) > 0
+    assert np.array_equal(grads[0], grads[1])

@@ -34,8 +34,16 @@ def objective64(args, freq, eps_box=None):
         mats.eps[:] = saved
 
 
-def test_the_fp64_adjoint_matches_central_differences():
-    args, freq = W.design_small(steps=260)
+@pytest.mark.parametrize("media", [False, True], ids=["plain", "debye-inductor-wire"])
+def test_the_fp64_adjoint_matches_central_differences(media):
+    """With `media` the step also carries the Debye ADE (its pole states are state), a lumped
+    inductor (its current is state) and thin-wire H corrections, each transposed in the adjoint."""
+    assert fd_error(media) < 1e-5
+
+
+def fd_error(media):
+    """max |adjoint - central difference| / max |central difference| over six seeded cells."""
+    args, freq = W.design_small(steps=260, media=media)
     grid, mats = args["grid"], args["mats"]
     desc = F.SimDesc(grid, mats, args["pec"], args["lumped"], args["ports"], args["resistive"], args["media"])
     ex = F.Extras(max_steps=args["steps"], decay_db=math.inf, probes=args["obs"])
@@ -64,7 +72,20 @@ def test_the_fp64_adjoint_matches_central_differences():
     scale = np.max(np.abs(fds))
     print(f"\nJ = {J:.6e}; adjoint {got}\n  central differences {fds}")
     assert scale > 0
-    assert np.max(np.abs(got - fds)) < 1e-5 * scale
+    return float(np.max(np.abs(got - fds)) / scale)
+
+
+@pytest.mark.parametrize("mutation", [15, 16, 17])
+def test_a_missing_media_transpose_is_caught(mutation):
+    """Planted in the adjoint only (the forward runs are faithful): 15 leaves the Debye pole states
+    out of the transposed step, 16 the thin wires, 17 flips the inductor's k_e."""
+    O.set_mutation(mutation)
+    try:
+        err = fd_error(True)
+    finally:
+        O.set_mutation(0)
+    print(f"planted {mutation}: {err:.2e}")
+    assert err > 1e-5  # ten times the bar of the faithful check, which reads ~1e-8
 
 
 def test_a_wrong_adjoint_sign_would_be_caught():
@@ -87,3 +108,12 @@ def test_the_design_box_may_not_touch_a_port():
     lo = (5, 12, 10)  # the port column sits at (7, 14)
     with pytest.raises(O.OracleError, match="port"):
         O.design_gradient(desc, 0, args["pulse"], 10, args["obs"], lo, args["hi"], np.zeros(10))
+
+
+def test_the_design_box_may_not_touch_a_thin_wire():
+    args, _ = W.design_small(steps=10, media=True)
+    desc = F.SimDesc(args["grid"], args["mats"], None, args["lumped"], args["ports"], [], args["media"])
+    hi = (17, 20, 14)  # the wire runs along z at (9, 21)
+    lo = (10, 11, 10)
+    with pytest.raises(O.OracleError, match="thin wire"):
+        O.design_gradient(desc, 0, args["pulse"], 10, args["obs"], lo, hi, np.zeros(10))
