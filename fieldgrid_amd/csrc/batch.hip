@@ -48,6 +48,8 @@ struct BatchRun {
     Src2 sr;
     Deb2 db;
     FixArgs fix2;
+    WireArgs wa;     // thin wires (wa.w NULL: none): the run's blocks take the wire march
+    SheetArgs sheet;  // resistive sheets, corrected after the march (sheet.n 0: none)
 };
 
 struct BatchBlock {
@@ -63,6 +65,42 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void k_march_batch(const BatchRun *_
     const BatchRun &r = runs[bb.run];
     march<TJ, PML, WIDE, LDS_SETS>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
                                    r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg);
+}
+
+// The same for runs with thin wires: the H correction of every copy of a wire's H node inside
+// the march, always 14 rows per block (k_march_wires of the single-run engine)
+template <int PML, bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * 16) void k_march_wires_batch(const BatchRun *__restrict__ runs,
+                                                               const BatchBlock *__restrict__ blocks, uint32_t par,
+                                                               uint32_t seg) {
+    const BatchBlock bb = blocks[blockIdx.x];
+    if (bb.bx == ~0u) return;
+    const BatchRun &r = runs[bb.run];
+    march<14, PML, WIDE, LDS_SETS, true>(r.g, MarchBlock{bb.bx, bb.by, bb.bz}, r.E[par], r.E[par ^ 1u], r.H[par],
+                                         r.H[par ^ 1u], r.psi[par], r.psi[par ^ 1u], seg, r.wa);
+}
+
+// Resistive sheets of the runs in `list` after their march (k_sheet of the single-run engine:
+// the face weights from the step's input E and new H, then the sheet current out of the new E),
+// one thread per sheet, or one thread for a run whose sheets share an edge
+__global__ __launch_bounds__(256) void k_sheet_batch(const BatchRun *__restrict__ runs, const uint32_t *__restrict__ list,
+                                                     uint32_t par) {
+    const BatchRun &r = runs[list[blockIdx.y]];
+    const SheetArgs &a = r.sheet;
+    const float *Eo = r.E[par], *Hn = r.H[par ^ 1u];
+    float *En = r.E[par ^ 1u];
+    const uint64_t t = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    auto one = [&](uint64_t q) {
+        const float *sh = a.sh + q * 16;
+        sheet_before(a, q, Eo[__float_as_uint(sh[0])], Hn[__float_as_uint(sh[1])], Hn[__float_as_uint(sh[2])]);
+        sheet_after(a, q, En);
+    };
+    if (a.serial) {
+        if (t == 0)
+            for (uint64_t q = 0; q < a.n; q++) one(q);
+    } else if (t < a.n) {
+        one(t);
+    }
 }
 
 constexpr int TJ2 = 12;  // rows per block of the two-step march
@@ -166,7 +204,7 @@ __global__ __launch_bounds__(256) void k_spectra_flat(const float *__restrict__ 
 
 // ---- host -----------------------------------------------------------------------------------
 
-constexpr int N_LISTS = 12;  // CPML class (3) x wide index (2) x LDS sets (2)
+constexpr int N_LISTS = 24;  // thin wires (2) x CPML class (3) x wide index (2) x LDS sets (2)
 
 struct Batch {
     fg_device *dev = nullptr;
@@ -180,8 +218,9 @@ struct Batch {
     uint32_t depth = 1, par = 0;
     bool pairs_ok = false;
     uint32_t seg2 = 32, seg2_base = 32;
-    DevBuf<uint32_t> fix_list, obs_list;
-    uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0;
+    DevBuf<uint32_t> fix_list, obs_list, sheet_list;
+    uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0, n_sheet = 0, sheet_gx = 0;
+    bool wires = false;  // some run has thin wires: 14-row tiles, one step per pass
     uint32_t tj = 14, seg = 32, chunk = 32, xcds = 8;
     uint32_t seg_base = 32;  // tuned planes per block; plan() halves it while too few blocks remain
     uint32_t cus = 256;
@@ -209,8 +248,8 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     b.drop_graph();
     std::vector<BatchBlock> L[N_LISTS], L2[N_LISTS];
     std::vector<MarchBlock> cls[3];
-    std::vector<uint32_t> fl, ol;
-    b.fix_gx = b.obs_gx = 0;
+    std::vector<uint32_t> fl, ol, sl;
+    b.fix_gx = b.obs_gx = b.sheet_gx = 0;
     // as runs finish, fewer blocks remain than the GPU holds: shorter segments (each block
     // marches fewer planes) keep ~6 blocks per CU in flight (3 for the two-step march, whose
     // blocks are twice as long-lived). The results do not depend on it.
@@ -242,10 +281,13 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
         if (!active[r]) continue;
         const Session &s = *b.runs[r];
         const int key = (s.geo.wide ? 2 : 0) + (s.geo.n_sets <= uint32_t(SETS_LDS) ? 1 : 0);
+        const int wl = s.wires.n ? 12 : 0;  // a run with thin wires takes the wire march's lists
         // one-step lists always (a two-step batch still takes an odd last step alone)
         march_classes(s, b.tj, b.seg, cls);
         for (int c = 0; c < 3; c++)
-            for (const MarchBlock &m : cls[c]) L[c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
+            for (const MarchBlock &m : cls[c]) L[wl + c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
+        if (s.sheet.n)
+            sl.push_back(r), b.sheet_gx = std::max(b.sheet_gx, s.sheet.serial ? 1u : uint32_t((s.sheet.n + 255) / 256));
         if (b.depth == 2) {
             march_classes2(s, TJ2, b.seg2, cls);
             for (int c = 0; c < 3; c++)
@@ -281,8 +323,10 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     // as runs finish, so the batch allocates them once
     b.fix_list.refill(fl.data(), fl.size(), b.stream);
     b.obs_list.refill(ol.data(), ol.size(), b.stream);
+    b.sheet_list.refill(sl.data(), sl.size(), b.stream);
     b.n_fix = uint32_t(fl.size());
     b.n_obs = uint32_t(ol.size());
+    b.n_sheet = uint32_t(sl.size());
     FG_CHECK(hipStreamSynchronize(b.stream));
 }
 
@@ -310,9 +354,26 @@ static void launch_march2(Batch &b, uint32_t par, uint32_t s_off) {
     }
 }
 
+template <int PML, bool W, bool L>
+static void launch_wires(Batch &b, int q, uint32_t par) {
+    hipLaunchKernelGGL((k_march_wires_batch<PML, W, L>), dim3(b.list_n[q]), dim3(64, 16), 0, b.stream, b.table.p,
+                       b.lists[q].p, par, b.seg);
+}
+
 template <int TJ>
 static void launch_march(Batch &b, uint32_t par) {
-    for (int q = 0; q < N_LISTS; q++) {
+    for (int q = 12; q < N_LISTS; q++) {  // runs with thin wires (b.tj is 14 whenever one exists)
+        if (!b.list_n[q]) continue;
+        switch (q - 12) {
+#define FG_CASE(Q, PML, W, L) \
+    case Q: launch_wires<PML, W, L>(b, q, par); break;
+            FG_CASE(0, 0, false, false) FG_CASE(1, 0, false, true) FG_CASE(2, 0, true, false) FG_CASE(3, 0, true, true)
+            FG_CASE(4, 1, false, false) FG_CASE(5, 1, false, true) FG_CASE(6, 1, true, false) FG_CASE(7, 1, true, true)
+            FG_CASE(8, 2, false, false) FG_CASE(9, 2, false, true) FG_CASE(10, 2, true, false) FG_CASE(11, 2, true, true)
+#undef FG_CASE
+        }
+    }
+    for (int q = 0; q < 12; q++) {
         if (!b.list_n[q]) continue;
         switch (q) {
 #define FG_CASE(Q, PML, W, L) \
@@ -329,6 +390,9 @@ static void launch_march(Batch &b, uint32_t par) {
 static void launch_step(Batch &b, uint32_t s_off, uint32_t par) {
     if (b.tj == 8) launch_march<8>(b, par);
     else launch_march<14>(b, par);
+    if (b.n_sheet)
+        hipLaunchKernelGGL(k_sheet_batch, dim3(b.sheet_gx, b.n_sheet), dim3(256), 0, b.stream, b.table.p,
+                           b.sheet_list.p, par);
     if (b.n_fix)
         hipLaunchKernelGGL(k_fix_batch, dim3(b.fix_gx, b.n_fix), dim3(256), 0, b.stream, b.table.p, b.fix_list.p, par,
                            s_off, 0u);
@@ -495,7 +559,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
     // step-fusion depth: two steps per pass only where every run allows it
     const bool ask_depth = (!t || !t->depth) && b.pairs_ok;
     b.depth = t && t->depth == 2 && b.pairs_ok ? 2 : 1;
-    b.tj = t && t->tj ? (t->tj == 8 ? 8 : 14) : 14;
+    b.tj = t && t->tj && !b.wires ? (t->tj == 8 ? 8 : 14) : 14;
     const uint32_t seg_in = t && t->seg ? std::min<uint32_t>(std::max<uint32_t>(t->seg, 1), SEG_MAX) : 32;
     b.seg = b.seg_base = b.seg2 = b.seg2_base = seg_in;
     b.chunk = t && t->chunk ? t->chunk : 32;
@@ -508,7 +572,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
         const std::vector<double> &v = it->second;
         const uint32_t depth = v.size() > 4 && uint32_t(v[4]) == 2 && b.pairs_ok ? 2 : 1;
         if (ask_depth) b.depth = depth;
-        if (ask_tj) b.tj = uint32_t(v[0]) == 8 ? 8 : 14;
+        if (ask_tj) b.tj = uint32_t(v[0]) == 8 && !b.wires ? 8 : 14;
         if (ask_seg) {
             const uint32_t sg = std::min<uint32_t>(std::max<uint32_t>(uint32_t(v[1]), 1), SEG_MAX);
             b.seg = b.seg_base = b.seg2 = b.seg2_base = sg;
@@ -526,7 +590,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
         for (uint32_t depth : {1u, 2u}) {
             if (!ask_depth && depth != b.depth) continue;
             for (uint32_t tj : {14u, 8u}) {
-                if (depth == 2 && tj != 14) continue;  // the two-step march has one tile height
+                if ((depth == 2 || b.wires) && tj != 14) continue;  // one tile height for these marches
                 if (depth == 1 && !ask_tj && tj != b.tj) continue;
                 // where two steps per pass are possible, one step per pass is only tried with
                 // 14-row tiles (8-row tiles re-read more halo and have not won there)
@@ -682,8 +746,6 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         bool all_pairs = true;
         for (uint32_t r = 0; r < count; r++) {
             if (!problems[r] || !extras[r]) throw HipError{"fg_run_batch: NULL problem or extras", FG_INVALID};
-            if (problems[r]->n_sheets || problems[r]->n_wires)
-                throw HipError{"fg_run_batch: resistive sheets and thin wires run through fg_run", FG_UNSUPPORTED};
         }
         {
             // the runs' sessions are built concurrently, so each one's own capacity check cannot
@@ -713,6 +775,9 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             t.fix2 = t.fix;
             if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
             if (!s.fuse2 || s.cap_start.n) all_pairs = false;  // the batch hands observers no captures
+            t.wa = WireArgs{s.wire_start.p, s.wire_ent.p, s.wires.n ? s.wires.p : nullptr};
+            t.sheet = s.sheet;
+            b.wires |= s.wires.n > 0;
         }
         b.pairs_ok = all_pairs && count > 0;
         b.table.upload(table.data(), table.size(), b.stream);

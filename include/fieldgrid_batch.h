@@ -60,8 +60,9 @@ typedef struct fg_spectra {
    problems/driven/pulses/extras/records are arrays of `count`; `spectra` is NULL or an array of
    `count` (an entry with nf = 0 is skipped). progress(user, steps, decay) is called on the
    calling thread whenever any run reaches a decay check, with the batch's step count and the
-   smallest decay among the runs still going; returning 0 cancels the whole batch. Resistive
-   sheets and thin wires give FG_UNSUPPORTED here (fg_run runs them). A run that goes unstable fails the
+   smallest decay among the runs still going; returning 0 cancels the whole batch. Runs with
+   resistive sheets or thin wires are batched too; they keep the batch at one step per pass and,
+   for wires, at 14-row tiles. A run that goes unstable fails the
    batch with FG_UNSTABLE and "run <r>: the FDTD run went unstable". The runs' device state is
    built on up to 8 host threads that end before the batch steps; an invalid problem fails the
    batch with its own status and message, as fg_run would. */

@@ -217,3 +217,27 @@ def test_a_given_segment_outside_the_tuners_list_is_timed_as_given(gpu):
     assert tune.used[1] == 20 and 0.0 < tune.step_ms < 1e3
     for q, r in enumerate(runs):
         _same(recs[q], F.run(gpu, r.sim, r.driven, r.pulse, r.extras), f"run {q}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tune", [F.Tune(seg=16, chunk=8), F.Tune(tj=8, seg=5, chunk=1), F.Tune()],
+                         ids=["seg16-graph8", "seg5-direct-asked-tj8", "autotuned"])
+def test_sheets_and_wires_in_a_batch_equal_single_runs(gpu, tune):
+    """Runs with resistive sheets and thin wires (the reference's everything() fixture, both ports
+    driven; and with the sheets alone and the wires alone) batched with plain sweep runs: the
+    wire runs take the wire march (14-row tiles whatever the caller asks), the sheet runs their
+    correction after the march, every run one step per pass; every record, the plane DFT and
+    NTFF accumulators included, bit-identical to a single fg_run."""
+    from .fixtures import everything, everything_extras
+    runs = []
+    for sheets, wires, driven in ((True, True, 0), (True, True, 1), (True, False, 0), (False, True, 1)):
+        sim = everything(sheets=sheets, wires=wires)
+        runs.append(F.BatchRun(sim, driven, F.Pulse(5e9, 5e9), everything_extras(sim, 150 + 17 * len(runs)),
+                               [3e9, 7e9]))
+    runs += small_sweep(count=2, steps=(90, 130))
+    recs, spectra = F.run_batch(gpu, runs, tune=tune, fields=True)
+    print(f"\n  used {tune.used}, step {tune.step_ms:.3f} ms")
+    assert tune.used[0] == 14 and tune.used[3] == 1
+    for q, r in enumerate(runs):
+        single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
+        _same(recs[q], single, f"run {q}")

@@ -154,15 +154,6 @@ def test_thin_wires_at_either_end_of_their_radius_stay_stable_on_the_device(gpu,
 
 
 @pytest.mark.gpu
-def test_a_batch_refuses_sheets_and_wires(gpu):
-    sim = everything()
-    job = F.BatchRun(sim, 0, F.Pulse(5e9, 5e9), F.Extras(max_steps=10, decay_db=math.inf), [])
-    with pytest.raises(abi.FieldgridError) as e:
-        F.run_batch(gpu, [job])
-    assert e.value.status == abi.FG_UNSUPPORTED
-
-
-@pytest.mark.gpu
 def test_a_run_stops_when_progress_says_so(gpu):
     """tests/fdtd.rs:176-187 on the device path."""
     sim = everything(sheets=False, wires=False)
