@@ -218,6 +218,7 @@ struct Batch {
     uint32_t depth = 1, par = 0;
     bool pairs_ok = false;
     uint32_t seg2 = 32, seg2_base = 32;
+    uint32_t seg2_cap = SEG_MAX;  // the two-step march's longest segment for every run's planes
     DevBuf<uint32_t> fix_list, obs_list, sheet_list;
     uint32_t n_fix = 0, n_obs = 0, fix_gx = 0, obs_gx = 0, n_sheet = 0, sheet_gx = 0;
     bool wires = false;  // some run has thin wires: 14-row tiles, one step per pass
@@ -254,7 +255,7 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     // marches fewer planes) keep ~6 blocks per CU in flight (3 for the two-step march, whose
     // blocks are twice as long-lived). The results do not depend on it.
     b.seg = b.seg_base;
-    b.seg2 = b.seg2_base;
+    b.seg2 = std::min(b.seg2_base, b.seg2_cap);
     if (b.depth == 2)
         for (;;) {
             uint64_t blocks = 0;
@@ -775,6 +776,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             t.fix2 = t.fix;
             if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
             if (!s.fuse2 || s.cap_start.n) all_pairs = false;  // the batch hands observers no captures
+            b.seg2_cap = std::min(b.seg2_cap, march2_seg_cap(march2_plane_bytes(s.n, s.pml)));
             t.wa = WireArgs{s.wire_start.p, s.wire_ent.p, s.wires.n ? s.wires.p : nullptr};
             t.sheet = s.sheet;
             b.wires |= s.wires.n > 0;

@@ -183,6 +183,50 @@ __device__ __forceinline__ void st(float *p, uint32_t byte_off, uint32_t bytes, 
 
 constexpr uint32_t OOB = 0x7FFFFFF0u;  // a byte offset past every resource: load 0, store dropped
 
+// A register move the compiler keeps where it stands (it neither drops nor moves an asm
+// statement): march2 moves each plane's loaded values out at the plane that uses them.
+__device__ __forceinline__ float vcopy(float x) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ uint32_t vcopy(uint32_t x) {
+    uint32_t r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Loads and stores through a resource built once (see march2's block resources).
+__device__ __forceinline__ float ldr(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(byte_off), 0, 0));
+}
+__device__ __forceinline__ uint32_t ldr32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, int(byte_off), 0, 0);
+}
+__device__ __forceinline__ uint32_t ldr16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, int(byte_off), 0, 0);
+}
+__device__ __forceinline__ void str(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, int(byte_off), 0, 0);
+}
+
+// march2 builds one resource per component array before its plane loop, based at the block's
+// first plane, and reaches plane p by adding (p - first) plane strides to the lane's offset.
+// Lanes outside the grid start at OOB2 instead of an in-plane offset. The host keeps a block's
+// planes within OOB2 bytes (march2_seg_cap), so every offset sum stays below 2^31, an in-grid
+// sum below the range and every other one past it.
+constexpr uint32_t OOB2 = 0x3FFFFFF0u;
+// Longest i segment march2 may take when the largest per-plane byte count of the arrays it
+// marches (a field plane, a z or y CPML psi row block) is `plane_bytes`: a block's resources
+// span at most seg + 4 planes, and its offsets reach one plane past them. 0 when not even one
+// plane fits; the host then takes one step per pass.
+__host__ __device__ constexpr uint32_t march2_seg_cap(uint64_t plane_bytes) {
+    return plane_bytes == 0 ? uint32_t(SEG_MAX)
+           : OOB2 / plane_bytes < 6 ? 0u
+           : OOB2 / plane_bytes - 5 < uint64_t(SEG_MAX) ? uint32_t(OOB2 / plane_bytes - 5)
+                                                        : uint32_t(SEG_MAX);
+}
+
 // psi = b*old + c*diff*raw (cpu.rs:78-99, 137-157), written to the new store at `off`.
 __device__ __forceinline__ float cpml_next(float *pn, uint32_t bytes, uint32_t off, float b, float c, float raw,
                                            float old, float diff) {
@@ -886,11 +930,11 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
     const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
     const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
     const uint32_t ob4 = off * 4u;
-    const uint32_t ldA = act ? ob4 : OOB, stO = own ? ob4 : OOB;
-    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t ldA = act ? ob4 : OOB2, stO = own ? ob4 : OOB2;
+    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB2;
     const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
     float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
-    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    uint32_t ozh = OOB2, oze = OOB2, oyh = OOB2, oye = OOB2;
     const uint32_t p2 = 2u * g.pml;
     const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
     if constexpr (PML) {
@@ -901,16 +945,12 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
         bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
         bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
         if (act) {
-            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
-            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB2, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB2;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB2, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB2;
         }
     }
     const uint32_t *cix = g.coef_index;
     const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
-    auto cidx = [&](uint64_t ub, uint32_t bytes) {
-        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
-        else return uint32_t(ld(cix16 + ub, ldC, bytes));
-    };
     auto coef = [&](uint32_t c) {
         if constexpr (LDS_SETS) return sets[c];
         else return g.coef_sets[c];
@@ -977,18 +1017,27 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 xseg[x] = cp.seg[xcap[lo * (TJ + 1)] + (x - xcoff[lo])];
             }
     }
-    // plane p of a field array through a resource that is empty when p is outside the grid
-    auto pbase = [&](const float *a, int64_t p) { return a + (p >= 0 && p < int64_t(g.n0) ? uint64_t(p) * plane : 0); };
-    auto pbytes = [&](int64_t p, uint32_t bytes) { return p >= 0 && p < int64_t(g.n0) ? bytes : 0u; };
+    // The block's field and coefficient-index resources, one per component array, based at plane
+    // tstart and ranged over its planes inside the grid (tstart .. min(i1 + 2, n0) - 1). Plane p
+    // sits (p - tstart) plane strides past a lane's in-plane offset, so the plane loop adds a
+    // uniform stride instead of rebuilding a resource per plane and array.
+    const uint32_t npl = min(i1 + 2, g.n0) - tstart;
+    __amdgpu_buffer_rsrc_t rEo[3], rHo[3], rEn[3], rHn[3], rC[3];
+    for (int c = 0; c < 3; c++) {
+        const uint64_t o = uint64_t(c) * nn + uint64_t(tstart) * plane;
+        rEo[c] = rsrc(Eo + o, npl * pl4), rHo[c] = rsrc(Ho + o, npl * pl4);
+        rEn[c] = rsrc(En + o, npl * pl4), rHn[c] = rsrc(Hn + o, npl * pl4);
+        if constexpr (WIDE) rC[c] = rsrc(cix + o, npl * plc);
+        else rC[c] = rsrc(cix16 + o, npl * plc);
+    }
+    auto cidx = [&](int c, uint32_t off_) {
+        if constexpr (WIDE) return ldr32(rC[c], off_);
+        else return ldr16(rC[c], off_);
+    };
 #define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
     // carried state: E0 of the current plane, step-n values of the previous plane, step-n+1 H
     // of the plane before, coefficient pairs and step-n psi of the previous plane
-    float ex0, ey0, ez0;
-    {
-        const float *p = pbase(Eo, tstart);
-        const uint32_t b = pbytes(tstart, pl4);
-        ex0 = ld(p, ldA, b), ey0 = ld(p + nn, ldA, b), ez0 = ld(p + 2 * nn, ldA, b);
-    }
+    float ex0 = ldr(rEo[0], ldA), ey0 = ldr(rEo[1], ldA), ez0 = ldr(rEo[2], ldA);
     {
         const int pb = (tstart + 1) & 1;  // the buffer the first iteration reads
         xr[0][pb][w + 1][0][lane] = ex0, xr[0][pb][w + 1][1][lane] = ez0;
@@ -1001,39 +1050,42 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
     // step-n psi of the previous plane (component, axis)
     float q_hxy = 0, q_hxz = 0, q_hyz = 0, q_hyx = 0, q_hzx = 0, q_hzy = 0;
     float q_exy = 0, q_exz = 0, q_eyz = 0, q_eyx = 0, q_ezx = 0, q_ezy = 0;
-    // the loads of plane t: E0(t+1), H0(t), coefficient indices and step n-1 psi. They are issued
-    // one iteration ahead (plane t+1's while plane t computes), so their latency overlaps a
-    // whole iteration instead of stalling between the barriers.
+    // the loads of plane t: E0(t+1), H0(t), coefficient indices and step n-1 psi, issued ahead
+    // of the plane that uses them (two planes ahead in the interior class, one in the CPML
+    // classes), so their latency overlaps whole iterations instead of stalling between the
+    // barriers. The vector memory counter counts loads and stores in issue order, so a wait for
+    // a plane's loads also waits for the stores issued before them, never for those after.
     struct Pl {
         float exn, eyn, ezn, hx0, hy0, hz0;
         uint32_t c0, c1, c2;
         float hxy, hxz, hyz, hyx, hzx, hzy, exy, exz, eyz, eyx, ezx, ezy;
     };
     auto load_plane = [&](uint32_t t, Pl &P) {
-        const uint64_t pb = uint64_t(t) * plane;
-        const float *pe = pbase(Eo, int64_t(t) + 1);
-        const uint32_t eb = pbytes(int64_t(t) + 1, pl4);
-        P.exn = ld(pe, ldA, eb), P.eyn = ld(pe + nn, ldA, eb), P.ezn = ld(pe + 2 * nn, ldA, eb);
-        const float *ph = pbase(Ho, t);
-        const uint32_t hb = pbytes(t, pl4);
-        P.hx0 = ld(ph, ldA, hb), P.hy0 = ld(ph + nn, ldA, hb), P.hz0 = ld(ph + 2 * nn, ldA, hb);
-        const uint32_t cbytes = pbytes(t, plc);
-        const uint64_t cb_ = t < g.n0 ? pb : 0;
-        P.c0 = cidx(cb_, cbytes), P.c1 = cidx(nn + cb_, cbytes), P.c2 = cidx(2 * nn + cb_, cbytes);
+        // plane t's H and indices and plane t+1's E. The plane loop issues these loads whether
+        // or not the plane is in the segment (a branch around them would blur the wait counts);
+        // a plane past the segment (t > i1) or past the grid reads from past every resource's
+        // range and gets 0.
+        const bool live = t <= i1;  // uniform
+        const uint32_t rel = live ? t - tstart : npl, oh = ldA + rel * pl4, oe = oh + pl4;
+        P.exn = ldr(rEo[0], oe), P.eyn = ldr(rEo[1], oe), P.ezn = ldr(rEo[2], oe);
+        P.hx0 = ldr(rHo[0], oh), P.hy0 = ldr(rHo[1], oh), P.hz0 = ldr(rHo[2], oh);
+        const uint32_t oc = ldC + rel * plc;
+        P.c0 = cidx(0, oc), P.c1 = cidx(1, oc), P.c2 = cidx(2, oc);
         P.hxy = P.hxz = P.hyz = P.hyx = P.hzx = P.hzy = 0.f;
         P.exy = P.exz = P.eyz = P.eyx = P.ezx = P.ezy = 0.f;
-        const uint32_t zb = t < g.n0 ? zbytes : 0u, yb = t < g.n0 ? ybytes : 0u, xb = t < g.n0 ? pl4 : 0u;
+        const bool in = live && t < g.n0;
+        const uint32_t zb = in ? zbytes : 0u, yb = in ? ybytes : 0u, xb = in ? pl4 : 0u;
         if constexpr (PZ) {
-            const uint64_t zrow = t < g.n0 ? uint64_t(t) * g.n1 * p2 : 0;
+            const uint64_t zrow = in ? uint64_t(t) * g.n1 * p2 : 0;
             P.hxz = ld(psio + PSI(1, 0, 2) + zrow, ozh, zb), P.hyz = ld(psio + PSI(1, 1, 2) + zrow, ozh, zb);
             P.exz = ld(psio + PSI(0, 0, 2) + zrow, oze, zb), P.eyz = ld(psio + PSI(0, 1, 2) + zrow, oze, zb);
         }
         if constexpr (PXY) {
-            const float *xv = xa[t - tstart];
+            const float *xv = xa[(live ? t : i1) - tstart];  // xa holds the segment's planes only
             const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
-            const uint64_t yrow = t < g.n0 ? uint64_t(t) * p2 * g.n2 : 0;
+            const uint64_t yrow = in ? uint64_t(t) * p2 * g.n2 : 0;
             const uint64_t xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
-            const uint32_t oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
+            const uint32_t oxh = shx >= 0 ? ldA : OOB2, oxe = sex >= 0 ? ldA : OOB2;
             P.hxy = ld(psio + PSI(1, 0, 1) + yrow, oyh, yb);
             P.hyx = ld(psio + PSI(1, 1, 0) + xhb, oxh, xb);
             P.hzx = ld(psio + PSI(1, 2, 0) + xhb, oxh, xb), P.hzy = ld(psio + PSI(1, 2, 1) + yrow, oyh, yb);
@@ -1042,13 +1094,10 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
             P.ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, xb), P.ezy = ld(psio + PSI(0, 2, 1) + yrow, oye, yb);
         }
     };
-    Pl nxt;
-    load_plane(tstart, nxt);
-    for (uint32_t t = tstart; t <= i1; t++) {
+    // one plane of the march: step n at plane t and step n+1 at plane t-1, from plane t's loads
+    auto plane_step = [&](const uint32_t t, const Pl &cur) __attribute__((always_inline)) {
         const int b = t & 1;
         const bool il = t + 1 < g.n0;
-        const Pl cur = nxt;
-        if (t < i1) load_plane(t + 1, nxt);  // uniform
         const float exn = cur.exn, eyn = cur.eyn, ezn = cur.ezn, hx0 = cur.hx0, hy0 = cur.hy0, hz0 = cur.hz0;
         const uint32_t c0 = cur.c0, c1 = cur.c1, c2 = cur.c2;
         const float *xv = xa[t - tstart];
@@ -1099,21 +1148,24 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
             float t1 = hv * iddy, t2 = hu * iddz;
             if constexpr (PXY) exy = bey * exy + cey * hv * rddy, t1 += exy;
             if constexpr (PZ) exz = bez * exz + cez * hu * rddz, t2 += exz;
-            nex = f0.y != 0.f ? f0.x * ex0 + f0.y * (t1 - t2) : ex0;
+            const float u = f0.x * ex0 + f0.y * (t1 - t2);  // computed for every lane: a select, not a branch
+            nex = f0.y != 0.f ? u : ex0;
         }
         {
             const float hv = nhx - hx_k, hu = nhz - h1z;
             float t1 = hv * iddz, t2 = hu * iddx;
             if constexpr (PZ) eyz = bez * eyz + cez * hv * rddz, t1 += eyz;
             if constexpr (PXY) eyx = bex * eyx + cex * hu * rddx, t2 += eyx;
-            ney = f1.y != 0.f ? f1.x * ey0 + f1.y * (t1 - t2) : ey0;
+            const float u = f1.x * ey0 + f1.y * (t1 - t2);
+            ney = f1.y != 0.f ? u : ey0;
         }
         {
             const float hv = nhy - h1y, hu = nhx - hx_j;
             float t1 = hv * iddx, t2 = hu * iddy;
             if constexpr (PXY) ezx = bex * ezx + cex * hv * rddx, t1 += ezx;
             if constexpr (PXY) ezy = bey * ezy + cey * hu * rddy, t2 += ezy;
-            nez = f2.y != 0.f ? f2.x * ez0 + f2.y * (t1 - t2) : ez0;
+            const float u = f2.x * ez0 + f2.y * (t1 - t2);
+            nez = f2.y != 0.f ? u : ez0;
         }
         // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
         // owner or halo), in the order of cpu.rs:284-297
@@ -1170,15 +1222,18 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
             }
         }
         // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)
+        // It runs on every plane but stores only where `two` holds (a branch around its stores
+        // would blur the wait counts of the loads in flight); elsewhere s is a plane the tables
+        // hold and the results go nowhere.
         const bool two = t >= 1 && t - 1 + 1 >= (i0 > 0 ? i0 : 1);
-        const uint32_t s = t - 1;  // plane of step n+1
+        const uint32_t s = two ? t - 1 : t;  // plane of step n+1
         const bool sl = t < g.n0;  // s+1 inside the grid (the i+1 neighbour exists)
-        float mhy = 0.f, mhz = 0.f;
-        if (two) {
+        float mhy, mhz;
+        {
             const float *xs = xa[s - tstart];
             const float sidx = xs[0], siddx = xs[1];
             float sbhx = 0, schx = 0, srdx = 0, sbex = 0, scex = 0, srddx = 0;
-            uint32_t soxh = OOB, soxe = OOB;
+            uint32_t soxh = OOB2, soxe = OOB2;
             uint64_t szrow = 0, syrow = 0, sxhb = 0, sxeb = 0;
             if constexpr (PZ) szrow = uint64_t(s) * g.n1 * p2;
             if constexpr (PXY) {
@@ -1187,10 +1242,11 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 sbex = xs[5], scex = xs[6], srddx = xs[7];
                 syrow = uint64_t(s) * p2 * g.n2;
                 sxhb = uint64_t(max(shx, 0)) * plane, sxeb = uint64_t(max(sex, 0)) * plane;
-                soxh = shx >= 0 ? ob4 : OOB, soxe = sex >= 0 ? ob4 : OOB;
+                soxh = shx >= 0 ? ob4 : OOB2, soxe = sex >= 0 ? ob4 : OOB2;
             }
-            const bool wr = own && s >= i0;  // this thread stores plane s
-            const uint32_t sto = wr ? stO : OOB;
+            const bool wr = own && two && s >= i0;  // this thread stores plane s
+            const uint32_t sto = wr ? stO : OOB2;
+            const uint32_t sfo = sto + (s - tstart) * pl4;  // plane s of the block's field resources
             const float ex_j2 = xr[2][b ^ 1][w + 2][0][lane], ez_j2 = xr[2][b ^ 1][w + 2][1][lane];
             const float ex_k2 = lane_next(e1x), ey_k2 = lane_next(e1y);
             float ohx, ohy, ohz;
@@ -1200,11 +1256,11 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 const bool v = jl && kl;
                 if constexpr (PXY) {
                     q_hxy = bhy * q_hxy + chy * ev * rdy, t1 += q_hxy;
-                    st(psin + PSI(1, 0, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hxy);
+                    st(psin + PSI(1, 0, 1) + syrow, v ? sto == OOB2 ? OOB2 : oyh : OOB2, ybytes, q_hxy);
                 }
                 if constexpr (PZ) {
                     q_hxz = bhz * q_hxz + chz * eu * rdz, t2 += q_hxz;
-                    st(psin + PSI(1, 0, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hxz);
+                    st(psin + PSI(1, 0, 2) + szrow, v ? sto == OOB2 ? OOB2 : ozh : OOB2, zbytes, q_hxz);
                 }
                 ohx = v ? h1x - g.k_mu * (t1 - t2) : h1x;
             }
@@ -1214,11 +1270,11 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 const bool v = kl && sl;
                 if constexpr (PZ) {
                     q_hyz = bhz * q_hyz + chz * ev * rdz, t1 += q_hyz;
-                    st(psin + PSI(1, 1, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hyz);
+                    st(psin + PSI(1, 1, 2) + szrow, v ? sto == OOB2 ? OOB2 : ozh : OOB2, zbytes, q_hyz);
                 }
                 if constexpr (PXY) {
                     q_hyx = sbhx * q_hyx + schx * eu * srdx, t2 += q_hyx;
-                    st(psin + PSI(1, 1, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hyx);
+                    st(psin + PSI(1, 1, 0) + sxhb, v ? sto == OOB2 ? OOB2 : soxh : OOB2, pl4, q_hyx);
                 }
                 ohy = v ? h1y - g.k_mu * (t1 - t2) : h1y;
             }
@@ -1228,35 +1284,32 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 const bool v = sl && jl;
                 if constexpr (PXY) {
                     q_hzx = sbhx * q_hzx + schx * ev * srdx, t1 += q_hzx;
-                    st(psin + PSI(1, 2, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hzx);
+                    st(psin + PSI(1, 2, 0) + sxhb, v ? sto == OOB2 ? OOB2 : soxh : OOB2, pl4, q_hzx);
                     q_hzy = bhy * q_hzy + chy * eu * rdy, t2 += q_hzy;
-                    st(psin + PSI(1, 2, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hzy);
+                    st(psin + PSI(1, 2, 1) + syrow, v ? sto == OOB2 ? OOB2 : oyh : OOB2, ybytes, q_hzy);
                 }
                 ohz = v ? h1z - g.k_mu * (t1 - t2) : h1z;
             }
-            {
-                float *q = Hn + uint64_t(s) * plane;
-                st(q, sto, pl4, ohx), st(q + nn, sto, pl4, ohy), st(q + 2 * nn, sto, pl4, ohz);
-            }
+            str(rHn[0], sfo, ohx), str(rHn[1], sfo, ohy), str(rHn[2], sfo, ohz);
             xr[3][b][w + 1][0][lane] = ohx, xr[3][b][w + 1][1][lane] = ohz;
             xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
             __syncthreads();
             const float hx_j2 = xr[3][b][w][0][lane], hz_j2 = xr[3][b][w][1][lane];
             const float hx_k2 = lane_prev(ohx), hy_k2 = lane_prev(ohy);
-            float *q = En + uint64_t(s) * plane;
             {
                 const float hv = ohz - hz_j2, hu = ohy - hy_k2;
                 float t1 = hv * iddy, t2 = hu * iddz;
                 const bool on = cf0.y != 0.f;
                 if constexpr (PXY) {
                     q_exy = bey * q_exy + cey * hv * rddy, t1 += q_exy;
-                    st(psin + PSI(0, 0, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_exy);
+                    st(psin + PSI(0, 0, 1) + syrow, on ? sto == OOB2 ? OOB2 : oye : OOB2, ybytes, q_exy);
                 }
                 if constexpr (PZ) {
                     q_exz = bez * q_exz + cez * hu * rddz, t2 += q_exz;
-                    st(psin + PSI(0, 0, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_exz);
+                    st(psin + PSI(0, 0, 2) + szrow, on ? sto == OOB2 ? OOB2 : oze : OOB2, zbytes, q_exz);
                 }
-                st(q, sto, pl4, on ? cf0.x * e1x + cf0.y * (t1 - t2) : e1x);
+                const float u = cf0.x * e1x + cf0.y * (t1 - t2);
+                str(rEn[0], sfo, on ? u : e1x);
             }
             {
                 const float hv = ohx - hx_k2, hu = ohz - h2z;
@@ -1264,13 +1317,14 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 const bool on = cf1.y != 0.f;
                 if constexpr (PZ) {
                     q_eyz = bez * q_eyz + cez * hv * rddz, t1 += q_eyz;
-                    st(psin + PSI(0, 1, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_eyz);
+                    st(psin + PSI(0, 1, 2) + szrow, on ? sto == OOB2 ? OOB2 : oze : OOB2, zbytes, q_eyz);
                 }
                 if constexpr (PXY) {
                     q_eyx = sbex * q_eyx + scex * hu * srddx, t2 += q_eyx;
-                    st(psin + PSI(0, 1, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_eyx);
+                    st(psin + PSI(0, 1, 0) + sxeb, on ? sto == OOB2 ? OOB2 : soxe : OOB2, pl4, q_eyx);
                 }
-                st(q + nn, sto, pl4, on ? cf1.x * e1y + cf1.y * (t1 - t2) : e1y);
+                const float u = cf1.x * e1y + cf1.y * (t1 - t2);
+                str(rEn[1], sfo, on ? u : e1y);
             }
             {
                 const float hv = ohy - h2y, hu = ohx - hx_j2;
@@ -1278,16 +1332,14 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
                 const bool on = cf2.y != 0.f;
                 if constexpr (PXY) {
                     q_ezx = sbex * q_ezx + scex * hv * srddx, t1 += q_ezx;
-                    st(psin + PSI(0, 2, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_ezx);
+                    st(psin + PSI(0, 2, 0) + sxeb, on ? sto == OOB2 ? OOB2 : soxe : OOB2, pl4, q_ezx);
                     q_ezy = bey * q_ezy + cey * hu * rddy, t2 += q_ezy;
-                    st(psin + PSI(0, 2, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_ezy);
+                    st(psin + PSI(0, 2, 1) + syrow, on ? sto == OOB2 ? OOB2 : oye : OOB2, ybytes, q_ezy);
                 }
-                st(q + 2 * nn, sto, pl4, on ? cf2.x * e1z + cf2.y * (t1 - t2) : e1z);
+                const float u = cf2.x * e1z + cf2.y * (t1 - t2);
+                str(rEn[2], sfo, on ? u : e1z);
             }
-            mhy = ohy, mhz = ohz;
-        } else {
-            xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
-            __syncthreads();
+            mhy = two ? ohy : 0.f, mhz = two ? ohz : 0.f;
         }
         // rotate: plane t becomes the previous plane
         h2y = mhy, h2z = mhz;
@@ -1297,6 +1349,56 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
         q_hxy = hxy, q_hxz = hxz, q_hyz = hyz, q_hyx = hyx, q_hzx = hzx, q_hzy = hzy;
         q_exy = exy, q_exz = exz, q_eyz = eyz, q_eyx = eyx, q_ezx = ezx, q_ezy = ezy;
         ex0 = exn, ey0 = eyn, ez0 = ezn;
+    };
+    // The interior class keeps two register sets of plane loads, each consumed two planes after
+    // it was issued. The loop is unrolled by two so that no set is copied into the other (a copy
+    // waits for its loads a plane early). The first two planes' loads are waited for before the
+    // loop, so the wait counts inside it follow the loop's own order of loads and stores.
+    // A set's values are moved out where its plane begins (vcopy: a move the compiler keeps in
+    // place), so the wait for its loads stands there and its registers are free for the loads
+    // two planes on; left to itself the compiler moves the newest set at the loop's end instead,
+    // which waits for loads issued a plane earlier.
+    auto settle = [&](const Pl &P) {
+        Pl c = P;
+        c.exn = vcopy(P.exn), c.eyn = vcopy(P.eyn), c.ezn = vcopy(P.ezn);
+        c.hx0 = vcopy(P.hx0), c.hy0 = vcopy(P.hy0), c.hz0 = vcopy(P.hz0);
+        c.c0 = vcopy(P.c0), c.c1 = vcopy(P.c1), c.c2 = vcopy(P.c2);
+        if constexpr (PZ) c.hxz = vcopy(P.hxz), c.hyz = vcopy(P.hyz), c.exz = vcopy(P.exz), c.eyz = vcopy(P.eyz);
+        if constexpr (PXY) {
+            c.hxy = vcopy(P.hxy), c.hyx = vcopy(P.hyx), c.hzx = vcopy(P.hzx), c.hzy = vcopy(P.hzy);
+            c.exy = vcopy(P.exy), c.eyx = vcopy(P.eyx), c.ezx = vcopy(P.ezx), c.ezy = vcopy(P.ezy);
+        }
+        return c;
+    };
+    if constexpr (PML == 0) {
+        Pl pa, pb;
+        load_plane(tstart, pa);
+        load_plane(tstart + 1, pb);  // i1 > tstart
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt at their maxima (no wait)
+        uint32_t t = tstart;
+        for (; t + 1 <= i1; t += 2) {
+            {
+                const Pl cur = settle(pa);
+                load_plane(t + 2, pa);
+                plane_step(t, cur);
+            }
+            {
+                const Pl cur = settle(pb);
+                load_plane(t + 3, pb);
+                plane_step(t + 1, cur);
+            }
+        }
+        if (t <= i1) plane_step(t, pa);  // an odd plane count's last plane
+    } else {
+        // the CPML classes carry up to 12 psi values more per plane: a second set would not fit
+        // in the registers a 1024-thread block leaves each lane, so they run one plane ahead
+        Pl nx;
+        load_plane(tstart, nx);
+        for (uint32_t t = tstart; t <= i1; t++) {
+            const Pl cur = nx;
+            load_plane(t + 1, nx);
+            plane_step(t, cur);
+        }
     }
 #undef PSI
 }

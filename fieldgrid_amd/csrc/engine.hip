@@ -927,12 +927,13 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
         }
     }
     const char *f2 = getenv("FIELDGRID_FUSE2");
+    const uint32_t seg2_cap = march2_seg_cap(march2_plane_bytes(P->n, P->pml));
     s.fuse2 = s.fused && !(f2 && f2[0] == '0') && !observed_debye && f.n_ind == 0 && !f.serial && P->n_sheets == 0 &&
-              P->n_wires == 0;
+              P->n_wires == 0 && seg2_cap > 0;
     if (s.fuse2) {
         s.seg2 = 32;
         if (const char *v = getenv("FIELDGRID_SEG2")) s.seg2 = std::max(1u, uint32_t(atoi(v)));
-        s.seg2 = std::min<uint32_t>(s.seg2, SEG_MAX);
+        s.seg2 = std::min<uint32_t>(s.seg2, seg2_cap);
         // Blocks are 1024 threads, one or two per CU, and a segment of S planes costs S + 3 plane
         // iterations: halve the segment only while the pass has fewer than 1.5 blocks per CU.
         // (Measured best: 32 planes at 768x768x384, 16 at 192x192x112 and 256^3 (32 within 3 %),

@@ -294,6 +294,12 @@ void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<Marc
 // The two-step march's block classes; `reach` more planes past a segment's end are read (the
 // two-step adjoint reads one: its first step's lamH at the last plane needs lamE one plane on).
 void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach = 0);
+// The largest per-plane byte count among the arrays the two-step march addresses plane by plane
+// (a field plane, one plane's rows of a z or y CPML psi slab): the argument of march2_seg_cap.
+inline uint64_t march2_plane_bytes(const uint32_t n[3], uint32_t pml) {
+    const uint64_t p2 = 2ull * pml, a = uint64_t(n[1]) * n[2], b = uint64_t(n[1]) * p2, c = p2 * n[2];
+    return 4ull * (a > b ? (a > c ? a : c) : (b > c ? b : c));
+}
 
 // Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute a block
 // list so that XCD x receives a contiguous run of the logical order (k tiles fastest, then j

@@ -300,18 +300,29 @@ def test_two_steps_per_launch_with_ntff_and_plane_dft_change_nothing(gpu, monkey
     _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"boxes with NTFF and plane DFT, seg2 {seg2}")
 
 
+# the two-step pass's dispatch: one launch over every block class (mixed), one launch per class
+# on the session stream (classes), or one per class on side streams forked from it (streams)
+_DISPATCH = {"mixed": {"FIELDGRID_MIX2": "1", "FIELDGRID_STREAMS": "0"},
+             "classes": {"FIELDGRID_MIX2": "0", "FIELDGRID_STREAMS": "0"},
+             "streams": {"FIELDGRID_MIX2": "0", "FIELDGRID_STREAMS": "1"}}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("seg2", ["32", "5", "1"])
-def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2):
+@pytest.mark.parametrize("seg2,dispatch", [("32", "mixed"), ("5", "mixed"), ("1", "mixed"),
+                                           ("5", "classes"), ("1", "streams")])
+def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2, dispatch):
     """march2 (two steps per pass, the intermediate step in registers) against one step per
     pass and against the FP32 oracle: every output bit-identical. Odd grid sizes, an odd
     check_every (an odd step count per chunk ends on a single step), segments of 1, 5 and 32
-    planes (segment starts at plane 0, 1 and further in), CPML 6, two ports, E and H probes."""
+    planes (segment starts at plane 0, 1 and further in), CPML 6, two ports, E and H probes,
+    under each of the pass's three dispatches."""
     sim = W.layered(dims=(70, 90, 66), dx=1e-3, pml=6, port_i=(20, 41))
     pulse = W.layered_pulse()
     ex = F.Extras(max_steps=301, decay_db=math.inf, check_every=77,
                   probes=[(2, 30, 40, 33), (4, 50, 52, 20), (0, 35, 45, 10)])
     monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    for k, v in _DISPATCH[dispatch].items():
+        monkeypatch.setenv(k, v)
     two = F.run(gpu, sim, 0, pulse, ex, fields=True)
     monkeypatch.setenv("FIELDGRID_FUSE2", "0")
     one = F.run(gpu, sim, 0, pulse, ex, fields=True)
@@ -319,4 +330,4 @@ def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2):
         a, b = getattr(two, name), getattr(one, name)
         print(f"  {name}: {int(np.sum(a != b))} of {a.size} differ")
         assert np.array_equal(a, b), name
-    _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"layered seg2 {seg2}")
+    _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"layered seg2 {seg2} {dispatch}")
