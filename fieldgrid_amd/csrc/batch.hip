@@ -750,9 +750,10 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         }
         {
             // the runs' sessions are built concurrently, so each one's own capacity check cannot
-            // see the others': check their sum first
+            // see the others': check their sum first, extras included
             double need = 0.0;
-            for (uint32_t r = 0; r < count; r++) need += session_bytes(problems[r], true);
+            for (uint32_t r = 0; r < count; r++)
+                need += session_bytes(problems[r], true) + session_extra_bytes(problems[r], extras[r]);
             size_t free_b = 0, total_b = 0;
             FG_CHECK(hipMemGetInfo(&free_b, &total_b));
             if (need > double(free_b)) {

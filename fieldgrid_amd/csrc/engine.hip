@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fieldgrid_diag.h"
 #include "host.h"
 
 namespace fg {
@@ -446,6 +447,27 @@ double session_bytes(const fg_problem *P, bool fused) {
     double psi = 0.0;  // 12 psi arrays, two per (component, other axis); a slab of 2 pml along it
     for (int a = 0; a < 3; a++) psi += pml ? 4.0 * (nn / P->n[a]) * 2.0 * pml : 0.0;
     return (fused ? 2.0 : 1.0) * (24.0 * nn + 4.0 * psi) + (P->coef_wide ? 12.0 : 6.0) * nn;
+}
+
+// Device bytes a session holds beyond session_bytes, bounded from above: Debye pole states and
+// the two-step march's Debye tables, port series and probe records, plane-DFT and NTFF
+// accumulators with their capture buffers, sheet and wire tables, and the small tables' arena
+// chunks with their rounding.
+double session_extra_bytes(const fg_problem *P, const fg_extras *X) {
+    const double n0 = P->n[0], n1 = P->n[1], n2 = P->n[2], nd = double(P->n_debye), np = P->n_poles;
+    double b = 0.0;
+    if (P->n_debye) {
+        b += nd * (sizeof(fg_debye_edge) + 4.0 * 3.0 * np + 4.0);  // edges, states (+ two-step J, E)
+        b += 4.0 * n0 * n1 + 12.0 * n2 * std::min(nd, n0 * n1);      // row slots, k maps
+    }
+    if (X) {
+        const double steps = double(X->max_steps), nf = X->nf;
+        b += 8.0 * steps * P->n_ports + 4.0 * steps * X->n_probes;    // V/I series, probes
+        if (X->plane_k >= 0) b += n0 * n1 * nf * 10.0 * 8.0 + n0 * n1 * 5.0 * 4.0 * 2.0;
+        b += double(X->n_patches) * (nf * 12.0 * 8.0 + 13.0 * 4.0 + 12.0 * 4.0 * 2.0);
+    }
+    b += 64.0 * double(P->n_sheets) + 64.0 * double(P->n_wires);
+    return b + double(8u << 20);  // arena chunks (2 MiB each) and allocation rounding
 }
 
 // Split the march blocks into those whose reach (their own nodes plus the one-node halo on

@@ -282,6 +282,8 @@ int guarded(F &&body) {
 // The device bytes of a session's fields, psi (both ping-pong copies when fused) and
 // coefficient index, in double so that no extent overflows it.
 double session_bytes(const fg_problem *P, bool fused);
+// the session's other device memory (Debye, records, observers, tables), bounded from above
+double session_extra_bytes(const fg_problem *P, const fg_extras *X);
 // One run's device state: validated, allocated and uploaded, fields at zero (engine.hip).
 Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse, const fg_extras *X);
 // Zero every field, psi, Debye, inductor-current and accumulator buffer and the step count on

@@ -250,11 +250,6 @@ void fg_session_free(fg_session *s);
 /* hipDeviceSynchronize on the handle's device. */
 int fg_device_synchronize(fg_device *dev);
 
-/* Bandwidth probe (no counterpart in the reference): copies `bytes` between two device
-   buffers `iterations` times and returns the achieved read+write GB/s. It calibrates the HBM
-   roofline and the profiler's FETCH_SIZE / WRITE_SIZE counters against a known traffic. */
-int fg_stream_copy(fg_device *dev, uint64_t bytes, uint32_t iterations, double *gbps);
-
 #ifdef __cplusplus
 }
 #endif
