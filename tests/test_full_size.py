@@ -67,6 +67,35 @@ def test_the_768x768x384_layered_run_two_steps_per_pass_change_nothing(gpu, monk
     _same(two, one, ("series", "e", "h"), "layered two vs one step per pass")
 
 
+def test_planes_near_the_offset_limit_march_two_steps_in_short_segments(gpu, monkeypatch):
+    """Planes of 5600 x 5600 nodes (125 MB per field component and plane). A two-step block
+    reaches its planes through 32-bit lane offsets from a resource based at its first plane, and
+    the host caps the segment so that those stay below 2^31: here at 3 planes, with offsets up
+    to ~2.08e9 bytes. 60 steps, two steps per pass against one, every output bit-identical, and
+    the pulse in every plane of E but the last (the boundary face, whose E the update leaves at
+    0, as the FP32 oracle does on a 12 x 60 x 60 version)."""
+    sim = W.layered(dims=(12, 5600, 5600), pml=3, port_i=(4, 7))
+    pulse = W.layered_pulse()
+    ex = F.Extras(max_steps=60, decay_db=math.inf, check_every=30)
+    assert _two_steps_per_pass(gpu, sim, pulse, ex)
+    two = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    monkeypatch.setenv("FIELDGRID_FUSE2", "0")
+    one = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    n = sim.dims()
+    planes = np.abs(two.e.reshape(3, n[0], n[1] * n[2])).max(axis=(0, 2))
+    print(f"\n12x5600x5600, 60 steps: largest |E| per plane {planes}")
+    assert np.all(planes[:-1] > 0)
+    _same(two, one, ("series", "e", "h"), "12x5600x5600 two vs one step per pass")
+
+
+def test_planes_past_the_offset_limit_take_one_step_per_pass(gpu):
+    """Planes of 7000 x 7000 nodes (196 MB): not even a one-plane segment keeps a two-step
+    block's offsets below 2^31, so the session marches one step per pass."""
+    sim = W.layered(dims=(8, 7000, 7000), pml=2, port_i=(3, 4))
+    ex = F.Extras(max_steps=4, decay_db=math.inf)
+    assert not _two_steps_per_pass(gpu, sim, W.layered_pulse(), ex)
+
+
 def test_the_256_cubed_boxes_run_matches_the_oracle(gpu):
     """240 steps: the pulse from the central port reaches the NTFF faces."""
     sim = W.boxes()
