@@ -331,3 +331,27 @@ def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2, dispatch):
         print(f"  {name}: {int(np.sum(a != b))} of {a.size} differ")
         assert np.array_equal(a, b), name
     _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"layered seg2 {seg2} {dispatch}")
+
+
+# (n0, n1, n2) around the march tiles' edges: k extents just under, at and over one and two
+# 60-lane tiles and 64 lanes, row counts at and just over multiples of the 12- and 14-row tiles,
+# plane counts under one segment and over two
+RAGGED = [(6, 7, 64), (9, 13, 61), (17, 25, 62), (33, 12, 121), (40, 14, 120), (12, 40, 59), (65, 30, 8),
+          (20, 61, 125)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seg2", [None, "1"], ids=["seg-default", "seg1"])
+@pytest.mark.parametrize("dims", RAGGED, ids=lambda d: "x".join(map(str, d)))
+def test_ragged_grids_match_the_fp32_path(gpu, monkeypatch, dims, seg2):
+    """Layered grids of ragged shapes with CPML 2 and two ports, 41 steps (an odd count: the
+    last step takes a pass of its own), on the default engine with its own segment length and
+    with one-plane segments: every field value bit-identical to the FP32 oracle, port series to
+    its last bit."""
+    if seg2:
+        monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    sim = W.layered(dims=dims, pml=2, port_i=(1, dims[0] // 2))
+    pulse = W.layered_pulse()
+    ex = F.Extras(max_steps=41, decay_db=math.inf, check_every=13)
+    rec = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    _exact(rec, O.run_cpu(sim.desc, 0, pulse, ex, 32), "x".join(map(str, dims)))
