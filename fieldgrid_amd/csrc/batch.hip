@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun 
     const MarchBlock mb{bb.bx, bb.by, bb.bz & 0x3FFFFFFFu};
     const int cls = PML >= 0 ? PML : int(bb.bz >> 30);
 #define FG_M2(C)                                                                                                    \
-    march2<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u], r.psi[par], \
+    march2_ahead<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u], r.psi[par], \
                                          r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{})
     if (cls == 2) FG_M2(2);
     else if (cls == 1) FG_M2(1);

@@ -930,6 +930,466 @@ __device__ __forceinline__ void march2(March2Lds<TJ, LDS_SETS, SIDE> &lds, const
     const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
     const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
     const uint32_t ob4 = off * 4u;
+    const uint32_t ldA = act ? ob4 : OOB, stO = own ? ob4 : OOB;
+    const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB;
+    const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);
+    float bhz = 0, chz = 0, rdz = 0, bez = 0, cez = 0, rddz = 0, bhy = 0, chy = 0, rdy = 0, bey = 0, cey = 0, rddy = 0;
+    uint32_t ozh = OOB, oze = OOB, oyh = OOB, oye = OOB;
+    const uint32_t p2 = 2u * g.pml;
+    const uint32_t zbytes = g.n1 * p2 * 4u, ybytes = p2 * g.n2 * 4u;
+    if constexpr (PML) {
+        const int shz = g.ax[2].slot_h[uk], sez = g.ax[2].slot_e[uk];
+        const int shy = g.ax[1].slot_h[uj], sey = g.ax[1].slot_e[uj];
+        bhz = g.ax[2].bh[uk], chz = g.ax[2].ch[uk], rdz = g.ax[2].raw_d[uk];
+        bez = g.ax[2].be[uk], cez = g.ax[2].ce[uk], rddz = g.ax[2].raw_dd[uk];
+        bhy = g.ax[1].bh[uj], chy = g.ax[1].ch[uj], rdy = g.ax[1].raw_d[uj];
+        bey = g.ax[1].be[uj], cey = g.ax[1].ce[uj], rddy = g.ax[1].raw_dd[uj];
+        if (act) {
+            ozh = shz >= 0 ? (uj * p2 + uint32_t(shz)) * 4u : OOB, oze = sez >= 0 ? (uj * p2 + uint32_t(sez)) * 4u : OOB;
+            oyh = shy >= 0 ? (uint32_t(shy) * g.n2 + uk) * 4u : OOB, oye = sey >= 0 ? (uint32_t(sey) * g.n2 + uk) * 4u : OOB;
+        }
+    }
+    const uint32_t *cix = g.coef_index;
+    const uint16_t *cix16 = reinterpret_cast<const uint16_t *>(g.coef_index);
+    auto cidx = [&](uint64_t ub, uint32_t bytes) {
+        if constexpr (WIDE) return ld(cix + ub, ldC, bytes);
+        else return uint32_t(ld(cix16 + ub, ldC, bytes));
+    };
+    auto coef = [&](uint32_t c) {
+        if constexpr (LDS_SETS) return sets[c];
+        else return g.coef_sets[c];
+    };
+    if constexpr (LDS_SETS)
+        for (uint32_t t = w * 64 + lane; t < g.n_sets; t += 64 * ROWS) sets[t] = g.coef_sets[t];
+    // step n runs on planes tstart..i1, step n+1 on planes max(i0,1)-1 .. i1-1 (the plane
+    // before i0 feeds the -x neighbour of i0's E update)
+    const uint32_t tstart = i0 >= 2 ? i0 - 2 : 0;
+    for (uint32_t t = w * 64 + lane; t + tstart <= i1; t += 64 * ROWS) {
+        const uint32_t q = min(tstart + t, g.n0 - 1);
+        xa[t][0] = g.ax[0].inv_d[q], xa[t][1] = g.ax[0].inv_dd[q];
+        if constexpr (PXY) {
+            xa[t][2] = g.ax[0].bh[q], xa[t][3] = g.ax[0].ch[q], xa[t][4] = g.ax[0].raw_d[q];
+            xa[t][5] = g.ax[0].be[q], xa[t][6] = g.ax[0].ce[q], xa[t][7] = g.ax[0].raw_dd[q];
+            xa[t][8] = __int_as_float(g.ax[0].slot_h[q]), xa[t][9] = __int_as_float(g.ax[0].slot_e[q]);
+        }
+    }
+    if (w < 2)
+        for (int x = 0; x < 4; x++)
+            for (int q = 0; q < 2; q++) xr[x][q][w * (ROWS + 1)][0][lane] = 0.f, xr[x][q][w * (ROWS + 1)][1][lane] = 0.f;
+    // The per-plane entries of the source, Debye-row and capture tables, staged in LDS before the
+    // march: the plane loop then reads them from LDS. Read from global memory inside the loop, a
+    // table entry's wait (vmcnt counts every vector memory access in issue order) would also
+    // wait for the plane loads issued one iteration ahead, once per plane.
+    //   xsrc[t - tstart]: sr.start[t] for the planes tstart .. i1 + 1 (clamped to n0);
+    //   xdeb[(t - tstart) * ROWS + w]: the Debye row slot of (t, row w), -1 outside the grid;
+    //   xcap[(t - i0) * (TJ + 1) + r]: cp.start of owned row r of owned plane t (r = TJ: the end),
+    //   xseg: the owned planes' capture segments, plane p's at xcoff[p] on, when they all fit.
+    constexpr uint32_t CAP_SEGS = March2Lds<TJ, LDS_SETS, SIDE>::CAP_SEGS;
+    auto &xsrc = lds.xsrc;
+    auto &xdeb = lds.xdeb;
+    auto &xcap = lds.xcap;
+    auto &xcoff = lds.xcoff;
+    auto &xseg = lds.xseg;
+    for (uint32_t t = w * 64 + lane; t + tstart <= i1 + 1; t += 64 * ROWS) xsrc[t] = sr.start[min(tstart + t, g.n0)];
+    bool seg_lds = false;
+    if constexpr (SIDE) {
+        const uint32_t np = i1 - tstart + 1, no = i1 - i0, j0 = mb.by * TJ;
+        for (uint32_t x = w * 64 + lane; x < np * ROWS; x += 64 * ROWS) {
+            const uint32_t t = tstart + x / ROWS;
+            const int jj = int(j0) + int(x % ROWS) - 2;
+            xdeb[x] = db.row && t < g.n0 && jj >= 0 && jj < int(g.n1) ? db.row[uint64_t(t) * g.n1 + uint32_t(jj)] : -1;
+        }
+        if (cp.start)
+            for (uint32_t x = w * 64 + lane; x < no * (TJ + 1); x += 64 * ROWS)
+                xcap[x] = cp.start[uint64_t(i0 + x / (TJ + 1)) * g.n1 + min(j0 + x % (TJ + 1), g.n1)];
+        __syncthreads();
+        if (cp.start && w == 0 && lane == 0) {
+            uint32_t o = 0;
+            for (uint32_t q = 0; q < no; q++) xcoff[q] = o, o += xcap[q * (TJ + 1) + TJ] - xcap[q * (TJ + 1)];
+            xcoff[no] = o;
+        }
+        __syncthreads();
+        seg_lds = cp.start && xcoff[no] <= CAP_SEGS;  // block-uniform
+        if (seg_lds)
+            for (uint32_t x = w * 64 + lane; x < xcoff[no]; x += 64 * ROWS) {
+                uint32_t lo = 0, hi = no;  // the owned plane of entry x: xcoff[lo] <= x < xcoff[hi]
+                while (hi - lo > 1) {
+                    const uint32_t m = (lo + hi) / 2;
+                    if (xcoff[m] <= x) lo = m;
+                    else hi = m;
+                }
+                xseg[x] = cp.seg[xcap[lo * (TJ + 1)] + (x - xcoff[lo])];
+            }
+    }
+    // plane p of a field array through a resource that is empty when p is outside the grid
+    auto pbase = [&](const float *a, int64_t p) { return a + (p >= 0 && p < int64_t(g.n0) ? uint64_t(p) * plane : 0); };
+    auto pbytes = [&](int64_t p, uint32_t bytes) { return p >= 0 && p < int64_t(g.n0) ? bytes : 0u; };
+#define PSI(e, c, a) g.psi_off[(e) * 9 + (c) * 3 + (a)]
+    // carried state: E0 of the current plane, step-n values of the previous plane, step-n+1 H
+    // of the plane before, coefficient pairs and step-n psi of the previous plane
+    float ex0, ey0, ez0;
+    {
+        const float *p = pbase(Eo, tstart);
+        const uint32_t b = pbytes(tstart, pl4);
+        ex0 = ld(p, ldA, b), ey0 = ld(p + nn, ldA, b), ez0 = ld(p + 2 * nn, ldA, b);
+    }
+    {
+        const int pb = (tstart + 1) & 1;  // the buffer the first iteration reads
+        xr[0][pb][w + 1][0][lane] = ex0, xr[0][pb][w + 1][1][lane] = ez0;
+    }
+    __syncthreads();
+    float h1x = 0, h1y = 0, h1z = 0;           // step-n H at the previous plane
+    float e1x = 0, e1y = 0, e1z = 0;           // step-n E at the previous plane
+    float h2y = 0, h2z = 0;                    // step-n+1 Hy, Hz at the plane before that
+    float2 cf0 = {0, 0}, cf1 = {0, 0}, cf2 = {0, 0};  // (ca, cb) of the previous plane
+    // step-n psi of the previous plane (component, axis)
+    float q_hxy = 0, q_hxz = 0, q_hyz = 0, q_hyx = 0, q_hzx = 0, q_hzy = 0;
+    float q_exy = 0, q_exz = 0, q_eyz = 0, q_eyx = 0, q_ezx = 0, q_ezy = 0;
+    // the loads of plane t: E0(t+1), H0(t), coefficient indices and step n-1 psi. They are issued
+    // one iteration ahead (plane t+1's while plane t computes), so their latency overlaps a
+    // whole iteration instead of stalling between the barriers.
+    struct Pl {
+        float exn, eyn, ezn, hx0, hy0, hz0;
+        uint32_t c0, c1, c2;
+        float hxy, hxz, hyz, hyx, hzx, hzy, exy, exz, eyz, eyx, ezx, ezy;
+    };
+    auto load_plane = [&](uint32_t t, Pl &P) {
+        const uint64_t pb = uint64_t(t) * plane;
+        const float *pe = pbase(Eo, int64_t(t) + 1);
+        const uint32_t eb = pbytes(int64_t(t) + 1, pl4);
+        P.exn = ld(pe, ldA, eb), P.eyn = ld(pe + nn, ldA, eb), P.ezn = ld(pe + 2 * nn, ldA, eb);
+        const float *ph = pbase(Ho, t);
+        const uint32_t hb = pbytes(t, pl4);
+        P.hx0 = ld(ph, ldA, hb), P.hy0 = ld(ph + nn, ldA, hb), P.hz0 = ld(ph + 2 * nn, ldA, hb);
+        const uint32_t cbytes = pbytes(t, plc);
+        const uint64_t cb_ = t < g.n0 ? pb : 0;
+        P.c0 = cidx(cb_, cbytes), P.c1 = cidx(nn + cb_, cbytes), P.c2 = cidx(2 * nn + cb_, cbytes);
+        P.hxy = P.hxz = P.hyz = P.hyx = P.hzx = P.hzy = 0.f;
+        P.exy = P.exz = P.eyz = P.eyx = P.ezx = P.ezy = 0.f;
+        const uint32_t zb = t < g.n0 ? zbytes : 0u, yb = t < g.n0 ? ybytes : 0u, xb = t < g.n0 ? pl4 : 0u;
+        if constexpr (PZ) {
+            const uint64_t zrow = t < g.n0 ? uint64_t(t) * g.n1 * p2 : 0;
+            P.hxz = ld(psio + PSI(1, 0, 2) + zrow, ozh, zb), P.hyz = ld(psio + PSI(1, 1, 2) + zrow, ozh, zb);
+            P.exz = ld(psio + PSI(0, 0, 2) + zrow, oze, zb), P.eyz = ld(psio + PSI(0, 1, 2) + zrow, oze, zb);
+        }
+        if constexpr (PXY) {
+            const float *xv = xa[t - tstart];
+            const int shx = __float_as_int(xv[8]), sex = __float_as_int(xv[9]);
+            const uint64_t yrow = t < g.n0 ? uint64_t(t) * p2 * g.n2 : 0;
+            const uint64_t xhb = uint64_t(max(shx, 0)) * plane, xeb = uint64_t(max(sex, 0)) * plane;
+            const uint32_t oxh = shx >= 0 ? ldA : OOB, oxe = sex >= 0 ? ldA : OOB;
+            P.hxy = ld(psio + PSI(1, 0, 1) + yrow, oyh, yb);
+            P.hyx = ld(psio + PSI(1, 1, 0) + xhb, oxh, xb);
+            P.hzx = ld(psio + PSI(1, 2, 0) + xhb, oxh, xb), P.hzy = ld(psio + PSI(1, 2, 1) + yrow, oyh, yb);
+            P.exy = ld(psio + PSI(0, 0, 1) + yrow, oye, yb);
+            P.eyx = ld(psio + PSI(0, 1, 0) + xeb, oxe, xb);
+            P.ezx = ld(psio + PSI(0, 2, 0) + xeb, oxe, xb), P.ezy = ld(psio + PSI(0, 2, 1) + yrow, oye, yb);
+        }
+    };
+    Pl nxt;
+    load_plane(tstart, nxt);
+    for (uint32_t t = tstart; t <= i1; t++) {
+        const int b = t & 1;
+        const bool il = t + 1 < g.n0;
+        const Pl cur = nxt;
+        if (t < i1) load_plane(t + 1, nxt);  // uniform
+        const float exn = cur.exn, eyn = cur.eyn, ezn = cur.ezn, hx0 = cur.hx0, hy0 = cur.hy0, hz0 = cur.hz0;
+        const uint32_t c0 = cur.c0, c1 = cur.c1, c2 = cur.c2;
+        const float *xv = xa[t - tstart];
+        const float idx_ = xv[0], iddx = xv[1];
+        float bhx = 0, chx = 0, rdx = 0, bex = 0, cex = 0, rddx = 0;
+        float hxy = cur.hxy, hxz = cur.hxz, hyz = cur.hyz, hyx = cur.hyx, hzx = cur.hzx, hzy = cur.hzy;
+        float exy = cur.exy, exz = cur.exz, eyz = cur.eyz, eyx = cur.eyx, ezx = cur.ezx, ezy = cur.ezy;
+        if constexpr (PXY) {
+            bhx = xv[2], chx = xv[3], rdx = xv[4];
+            bex = xv[5], cex = xv[6], rddx = xv[7];
+        }
+        // ---- step n, H at plane t (the +1 neighbours of E0: j+1 from LDS, k+1 from the next lane)
+        const float ex_j = xr[0][b ^ 1][w + 2][0][lane], ez_j = xr[0][b ^ 1][w + 2][1][lane];
+        const float ex_k = lane_next(ex0), ey_k = lane_next(ey0);
+        float nhx, nhy, nhz;
+        {
+            const float ev = ez_j - ez0, eu = ey_k - ey0;
+            float t1 = ev * idy, t2 = eu * idz;
+            if constexpr (PXY) hxy = bhy * hxy + chy * ev * rdy, t1 += hxy;
+            if constexpr (PZ) hxz = bhz * hxz + chz * eu * rdz, t2 += hxz;
+            nhx = jl && kl ? hx0 - g.k_mu * (t1 - t2) : hx0;
+        }
+        {
+            const float ev = ex_k - ex0, eu = ezn - ez0;
+            float t1 = ev * idz, t2 = eu * idx_;
+            if constexpr (PZ) hyz = bhz * hyz + chz * ev * rdz, t1 += hyz;
+            if constexpr (PXY) hyx = bhx * hyx + chx * eu * rdx, t2 += hyx;
+            nhy = kl && il ? hy0 - g.k_mu * (t1 - t2) : hy0;
+        }
+        {
+            const float ev = eyn - ey0, eu = ex_j - ex0;
+            float t1 = ev * idx_, t2 = eu * idy;
+            if constexpr (PXY) hzx = bhx * hzx + chx * ev * rdx, t1 += hzx;
+            if constexpr (PXY) hzy = bhy * hzy + chy * eu * rdy, t2 += hzy;
+            nhz = il && jl ? hz0 - g.k_mu * (t1 - t2) : hz0;
+        }
+        xr[1][b][w + 1][0][lane] = nhx, xr[1][b][w + 1][1][lane] = nhz;
+        xr[0][b][w + 1][0][lane] = exn, xr[0][b][w + 1][1][lane] = ezn;
+        __syncthreads();
+        // ---- step n, E at plane t (the -1 neighbours of step-n H: j-1 from LDS, k-1 from the
+        // previous lane, i-1 from the previous iteration)
+        const float hx_j = xr[1][b][w][0][lane], hz_j = xr[1][b][w][1][lane];
+        const float hx_k = lane_prev(nhx), hy_k = lane_prev(nhy);
+        const float2 f0 = coef(c0), f1 = coef(c1), f2 = coef(c2);
+        float nex, ney, nez;
+        {
+            const float hv = nhz - hz_j, hu = nhy - hy_k;
+            float t1 = hv * iddy, t2 = hu * iddz;
+            if constexpr (PXY) exy = bey * exy + cey * hv * rddy, t1 += exy;
+            if constexpr (PZ) exz = bez * exz + cez * hu * rddz, t2 += exz;
+            nex = f0.y != 0.f ? f0.x * ex0 + f0.y * (t1 - t2) : ex0;
+        }
+        {
+            const float hv = nhx - hx_k, hu = nhz - h1z;
+            float t1 = hv * iddz, t2 = hu * iddx;
+            if constexpr (PZ) eyz = bez * eyz + cez * hv * rddz, t1 += eyz;
+            if constexpr (PXY) eyx = bex * eyx + cex * hu * rddx, t2 += eyx;
+            ney = f1.y != 0.f ? f1.x * ey0 + f1.y * (t1 - t2) : ey0;
+        }
+        {
+            const float hv = nhy - h1y, hu = nhx - hx_j;
+            float t1 = hv * iddx, t2 = hu * iddy;
+            if constexpr (PXY) ezx = bex * ezx + cex * hv * rddx, t1 += ezx;
+            if constexpr (PXY) ezy = bey * ezy + cey * hu * rddy, t2 += ezy;
+            nez = f2.y != 0.f ? f2.x * ez0 + f2.y * (t1 - t2) : ez0;
+        }
+        // step n's Debye ADE, then its port sources, on this plane (every copy of the edge,
+        // owner or halo), in the order of cpu.rs:284-297
+        const int drow = SIDE ? xdeb[(t - tstart) * ROWS + w] : -1;  // wave-uniform
+        int32_t dt3[3] = {-1, -1, -1};
+        if (SIDE && drow >= 0) {
+            const int32_t *dm = db.map + (uint64_t(uint32_t(drow)) * g.n2 + uk) * 3;
+            for (int c = 0; c < 3; c++) dt3[c] = kin ? dm[c] : -1;
+            for (int c = 0; c < 3; c++) {
+                const int32_t di = dt3[c];
+                if (di < 0) continue;
+                const fg_debye_edge d = fx.debye[di];
+                const float cb = (c == 0 ? f0 : c == 1 ? f1 : f2).y;
+                const float *tab = fx.dtab + size_t(d.set) * fx.np * 2;
+                float s0 = 0.f;
+                for (uint32_t q = 0; q < fx.np; q++) s0 += 0.5f * (1.0f + tab[2 * q]) * fx.dstate[q * fx.n_debye + di];
+                float &e = c == 0 ? nex : c == 1 ? ney : nez;
+                const float next = e - cb * s0;
+                e = next;
+                const float de = next - (c == 0 ? ex0 : c == 1 ? ey0 : ez0);
+                if (own && t >= i0 && t < i1)
+                    for (uint32_t q = 0; q < fx.np; q++)
+                        db.jn[q * fx.n_debye + di] = tab[2 * q] * fx.dstate[q * fx.n_debye + di] + d.dd * tab[2 * q + 1] * de;
+            }
+        }
+        if (t < g.n0) {
+            const uint32_t s0 = xsrc[t - tstart], s1 = xsrc[t - tstart + 1];
+            for (uint32_t q = s0; q < s1; q++) {
+                const uint4 e = sr.ent[q];
+                if (int(e.x) == j && int(e.y) == k) {
+                    const float d = __uint_as_float(e.w) * pl;
+                    if (e.z == 0) nex -= d;
+                    else if (e.z == 1) ney -= d;
+                    else nez -= d;
+                }
+            }
+        }
+        if (SIDE && drow >= 0 && own && t >= i0 && t < i1)
+            for (int c = 0; c < 3; c++)
+                if (dt3[c] >= 0) db.eo[dt3[c]] = c == 0 ? nex : c == 1 ? ney : nez;
+        // step n's final values at the observed points of this row, by their owner
+        if (SIDE && cp.start && t >= i0 && t < i1 && w >= 2 && w < 2 + TJ && jin) {  // wave-uniform
+            const uint32_t *hp = xcap + (t - i0) * (TJ + 1);
+            const uint32_t q0 = hp[w - 2], q1 = hp[w - 1], at0 = xcoff[t - i0] - hp[0];
+            const bool mine = own;
+            for (uint32_t q = q0; q < q1; q++) {
+                const uint4 sg = seg_lds ? xseg[q + at0] : cp.seg[q];
+                if (mine && uk >= sg.x && uk < sg.y) {
+                    uint32_t at = sg.w + (uk - sg.x) * uint32_t(__popc(sg.z));
+                    const float v[6] = {nex, ney, nez, nhx, nhy, nhz};
+                    for (int c = 0; c < 6; c++)
+                        if ((sg.z >> c) & 1u) cp.buf[at++] = v[c];
+                }
+            }
+        }
+        // ---- step n+1 at plane t-1: H from the step-n E of planes t-1 (carried) and t (+x)
+        const bool two = t >= 1 && t - 1 + 1 >= (i0 > 0 ? i0 : 1);
+        const uint32_t s = t - 1;  // plane of step n+1
+        const bool sl = t < g.n0;  // s+1 inside the grid (the i+1 neighbour exists)
+        float mhy = 0.f, mhz = 0.f;
+        if (two) {
+            const float *xs = xa[s - tstart];
+            const float sidx = xs[0], siddx = xs[1];
+            float sbhx = 0, schx = 0, srdx = 0, sbex = 0, scex = 0, srddx = 0;
+            uint32_t soxh = OOB, soxe = OOB;
+            uint64_t szrow = 0, syrow = 0, sxhb = 0, sxeb = 0;
+            if constexpr (PZ) szrow = uint64_t(s) * g.n1 * p2;
+            if constexpr (PXY) {
+                const int shx = __float_as_int(xs[8]), sex = __float_as_int(xs[9]);
+                sbhx = xs[2], schx = xs[3], srdx = xs[4];
+                sbex = xs[5], scex = xs[6], srddx = xs[7];
+                syrow = uint64_t(s) * p2 * g.n2;
+                sxhb = uint64_t(max(shx, 0)) * plane, sxeb = uint64_t(max(sex, 0)) * plane;
+                soxh = shx >= 0 ? ob4 : OOB, soxe = sex >= 0 ? ob4 : OOB;
+            }
+            const bool wr = own && s >= i0;  // this thread stores plane s
+            const uint32_t sto = wr ? stO : OOB;
+            const float ex_j2 = xr[2][b ^ 1][w + 2][0][lane], ez_j2 = xr[2][b ^ 1][w + 2][1][lane];
+            const float ex_k2 = lane_next(e1x), ey_k2 = lane_next(e1y);
+            float ohx, ohy, ohz;
+            {
+                const float ev = ez_j2 - e1z, eu = ey_k2 - e1y;
+                float t1 = ev * idy, t2 = eu * idz;
+                const bool v = jl && kl;
+                if constexpr (PXY) {
+                    q_hxy = bhy * q_hxy + chy * ev * rdy, t1 += q_hxy;
+                    st(psin + PSI(1, 0, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hxy);
+                }
+                if constexpr (PZ) {
+                    q_hxz = bhz * q_hxz + chz * eu * rdz, t2 += q_hxz;
+                    st(psin + PSI(1, 0, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hxz);
+                }
+                ohx = v ? h1x - g.k_mu * (t1 - t2) : h1x;
+            }
+            {
+                const float ev = ex_k2 - e1x, eu = nez - e1z;
+                float t1 = ev * idz, t2 = eu * sidx;
+                const bool v = kl && sl;
+                if constexpr (PZ) {
+                    q_hyz = bhz * q_hyz + chz * ev * rdz, t1 += q_hyz;
+                    st(psin + PSI(1, 1, 2) + szrow, v ? sto == OOB ? OOB : ozh : OOB, zbytes, q_hyz);
+                }
+                if constexpr (PXY) {
+                    q_hyx = sbhx * q_hyx + schx * eu * srdx, t2 += q_hyx;
+                    st(psin + PSI(1, 1, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hyx);
+                }
+                ohy = v ? h1y - g.k_mu * (t1 - t2) : h1y;
+            }
+            {
+                const float ev = ney - e1y, eu = ex_j2 - e1x;
+                float t1 = ev * sidx, t2 = eu * idy;
+                const bool v = sl && jl;
+                if constexpr (PXY) {
+                    q_hzx = sbhx * q_hzx + schx * ev * srdx, t1 += q_hzx;
+                    st(psin + PSI(1, 2, 0) + sxhb, v ? sto == OOB ? OOB : soxh : OOB, pl4, q_hzx);
+                    q_hzy = bhy * q_hzy + chy * eu * rdy, t2 += q_hzy;
+                    st(psin + PSI(1, 2, 1) + syrow, v ? sto == OOB ? OOB : oyh : OOB, ybytes, q_hzy);
+                }
+                ohz = v ? h1z - g.k_mu * (t1 - t2) : h1z;
+            }
+            {
+                float *q = Hn + uint64_t(s) * plane;
+                st(q, sto, pl4, ohx), st(q + nn, sto, pl4, ohy), st(q + 2 * nn, sto, pl4, ohz);
+            }
+            xr[3][b][w + 1][0][lane] = ohx, xr[3][b][w + 1][1][lane] = ohz;
+            xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
+            __syncthreads();
+            const float hx_j2 = xr[3][b][w][0][lane], hz_j2 = xr[3][b][w][1][lane];
+            const float hx_k2 = lane_prev(ohx), hy_k2 = lane_prev(ohy);
+            float *q = En + uint64_t(s) * plane;
+            {
+                const float hv = ohz - hz_j2, hu = ohy - hy_k2;
+                float t1 = hv * iddy, t2 = hu * iddz;
+                const bool on = cf0.y != 0.f;
+                if constexpr (PXY) {
+                    q_exy = bey * q_exy + cey * hv * rddy, t1 += q_exy;
+                    st(psin + PSI(0, 0, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_exy);
+                }
+                if constexpr (PZ) {
+                    q_exz = bez * q_exz + cez * hu * rddz, t2 += q_exz;
+                    st(psin + PSI(0, 0, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_exz);
+                }
+                st(q, sto, pl4, on ? cf0.x * e1x + cf0.y * (t1 - t2) : e1x);
+            }
+            {
+                const float hv = ohx - hx_k2, hu = ohz - h2z;
+                float t1 = hv * iddz, t2 = hu * siddx;
+                const bool on = cf1.y != 0.f;
+                if constexpr (PZ) {
+                    q_eyz = bez * q_eyz + cez * hv * rddz, t1 += q_eyz;
+                    st(psin + PSI(0, 1, 2) + szrow, on ? sto == OOB ? OOB : oze : OOB, zbytes, q_eyz);
+                }
+                if constexpr (PXY) {
+                    q_eyx = sbex * q_eyx + scex * hu * srddx, t2 += q_eyx;
+                    st(psin + PSI(0, 1, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_eyx);
+                }
+                st(q + nn, sto, pl4, on ? cf1.x * e1y + cf1.y * (t1 - t2) : e1y);
+            }
+            {
+                const float hv = ohy - h2y, hu = ohx - hx_j2;
+                float t1 = hv * siddx, t2 = hu * iddy;
+                const bool on = cf2.y != 0.f;
+                if constexpr (PXY) {
+                    q_ezx = sbex * q_ezx + scex * hv * srddx, t1 += q_ezx;
+                    st(psin + PSI(0, 2, 0) + sxeb, on ? sto == OOB ? OOB : soxe : OOB, pl4, q_ezx);
+                    q_ezy = bey * q_ezy + cey * hu * rddy, t2 += q_ezy;
+                    st(psin + PSI(0, 2, 1) + syrow, on ? sto == OOB ? OOB : oye : OOB, ybytes, q_ezy);
+                }
+                st(q + 2 * nn, sto, pl4, on ? cf2.x * e1z + cf2.y * (t1 - t2) : e1z);
+            }
+            mhy = ohy, mhz = ohz;
+        } else {
+            xr[2][b][w + 1][0][lane] = nex, xr[2][b][w + 1][1][lane] = nez;
+            __syncthreads();
+        }
+        // rotate: plane t becomes the previous plane
+        h2y = mhy, h2z = mhz;
+        h1x = nhx, h1y = nhy, h1z = nhz;
+        e1x = nex, e1y = ney, e1z = nez;
+        cf0 = f0, cf1 = f1, cf2 = f2;
+        q_hxy = hxy, q_hxz = hxz, q_hyz = hyz, q_hyx = hyx, q_hzx = hzx, q_hzy = hzy;
+        q_exy = exy, q_exz = exz, q_eyz = eyz, q_eyx = eyx, q_ezx = ezx, q_ezy = ezy;
+        ex0 = exn, ey0 = eyn, ez0 = ezn;
+    }
+#undef PSI
+}
+
+// march2_ahead computes exactly what march2 computes (bit for bit), with the plane loop built for
+// the memory pipeline rather than for register and scalar economy:
+//  - the field and coefficient-index planes come through resources built once per block in the
+//    interior class without side work (block-wide resources, 60 scalar registers), per plane
+//    elsewhere;
+//  - the interior classes issue each plane's loads two planes ahead (two register sets, the loop
+//    unrolled by two), the CPML classes one;
+//  - every load and store is issued unconditionally (out of range where it does not apply), and
+//    each plane's loaded values are moved out where the plane begins, so the compiler's waits
+//    stay exact and never drain the loads in flight; the second step also runs on the
+//    segment's first planes, storing nothing there;
+//  - the E updates are selects.
+// Measured A/B on one box: it takes the 768x768x384 layered interior launch from 4.656 to 4.612
+// ms per two steps, and the batched sweep from 5.59 to 5.43 s; march2 stays ahead where side
+// work and short segments dominate (the 256^3 boxes run: 0.333 against 0.349 ms per step; the
+// sweep's single runs: 10.23 against 10.79 s). The engine runs march2_ahead for the interior
+// class without side work, the batch for every class; march2 runs everything else.
+// SIDE: the first step has side work (Debye ADE through db, captures through cp; either may be
+// absent, NULL tables)
+template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool SIDE>
+__device__ __forceinline__ void march2_ahead(March2Lds<TJ, LDS_SETS, SIDE> &lds, const Geo &g, const MarchBlock mb,
+                                       const float *__restrict__ Eo,
+                                       float *__restrict__ En, const float *__restrict__ Ho, float *__restrict__ Hn,
+                                       const float *__restrict__ psio, float *__restrict__ psin, uint32_t seg,
+                                       const Src2 &sr, float pl, const FixArgs &fx, const Deb2 &db,
+                                       const Cap2 &cp) {
+    constexpr int ROWS = TJ + 4;
+    constexpr bool PZ = PML >= 1, PXY = PML == 2;
+    auto &xr = lds.xr;
+    auto &sets = lds.sets;
+    auto &xa = lds.xa;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k = int(mb.bx) * OWN_K2 + lane - 2;
+    const int j = int(mb.by) * TJ + w - 2;
+    const uint32_t i0 = mb.bz * seg, i1 = min(i0 + seg, g.n0);
+    const bool kin = k >= 0 && k < int(g.n2), jin = j >= 0 && j < int(g.n1);
+    const bool act = kin && jin;
+    const bool own = act && lane >= 2 && lane < 2 + OWN_K2 && w >= 2 && w < 2 + TJ;
+    const uint32_t uk = kin ? uint32_t(k) : 0u, uj = jin ? uint32_t(j) : 0u;
+    const uint64_t nn = g.nn, plane = uint64_t(g.n1) * g.n2;
+    const uint32_t off = uj * g.n2 + uk;
+    const bool kl = uk + 1 < g.n2, jl = uj + 1 < g.n1;
+    const float idz = g.ax[2].inv_d[uk], iddz = g.ax[2].inv_dd[uk];
+    const float idy = g.ax[1].inv_d[uj], iddy = g.ax[1].inv_dd[uj];
+    const uint32_t ob4 = off * 4u;
     const uint32_t ldA = act ? ob4 : OOB2, stO = own ? ob4 : OOB2;
     const uint32_t ldC = act ? off * (WIDE ? 4u : 2u) : OOB2;
     const uint32_t pl4 = uint32_t(plane) * 4u, plc = uint32_t(plane) * (WIDE ? 4u : 2u);

@@ -282,7 +282,8 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
     __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
-    march2<TJ, PML, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    if constexpr (PML == 0 && !DEB) march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else march2<TJ, PML, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
 // The same for a block list that mixes the CPML classes: the class is in the two top bits of
@@ -306,6 +307,7 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2_mix(Geo g, const March
     __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
     if (cls == 2) march2<TJ, 2, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
     else if (cls == 1) march2<TJ, 1, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else if constexpr (!DEB) march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
     else march2<TJ, 0, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
