@@ -153,7 +153,7 @@ def sweep_leg(args):
     gpu.synchronize()
     seq_s = time.perf_counter() - t0
     log(f"sweep: sequential {seq_s:.2f} s; batched run")
-    tune = F.Tune(chunk=args.graph_chunk)
+    tune = F.Tune(chunk=args.graph_chunk, launch=args.launch)
     t0 = time.perf_counter()
     recs, spectra = F.run_batch(gpu, runs, tune=tune)
     gpu.synchronize()
@@ -165,13 +165,19 @@ def sweep_leg(args):
     alg = sum(W.bytes_per_node_step(r.sim, None, core=True) * int(np.prod(r.sim.dims())) for r in runs)
     achieved = alg / (tune.step_ms * 1e-3) / 1e9 if tune.step_ms > 0 else None
     depth = tune.used[3]
-    kernel = ("k_march2_batch<-1> (class-mixed lists, two steps per pass) + k_fix_batch + k_observe_pair_batch"
-              if depth == 2 else "k_march_batch (one launch per class, one step per pass) + k_fix_batch + k_observe_batch")
+    if depth == 2 and tune.used_launch == 2:
+        kernel = ("k_march2_persist (persistent launches of many two-step passes: each run goes on to its next "
+                  "pass when its previous one is done; the pass's last block runs the fixes and observers)")
+    elif depth == 2:
+        kernel = "k_march2_batch<-1> (class-mixed lists, two steps per pass) + k_fix_batch + k_observe_pair_batch"
+    else:
+        kernel = "k_march_batch (one launch per class, one step per pass) + k_fix_batch + k_observe_batch"
     obj = {"runs": len(runs), "node_updates": node_steps, "batched_s": round(bat_s, 3), "sequential_s": round(seq_s, 3),
            "speedup": round(seq_s / bat_s, 3), "batched_mcells": round(node_steps / bat_s / 1e6, 2),
            "sequential_mcells": round(node_steps / seq_s / 1e6, 2), "records_identical": same,
            "build_s": round(build_s, 2),
            "tune": {"tj": tune.used[0], "seg": tune.used[1], "chunk": tune.used[2], "depth": depth,
+                    "launch": "persistent" if tune.used_launch == 2 else "per pass",
                     "tune_ms": round(tune.tune_ms, 1), "kept_steps": tune.kept_steps, "from_cache": tune.from_cache},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
@@ -375,6 +381,8 @@ def main():
                     help="layered: the legs added to the line (sweep, adjoint; 'none' for the headline alone)")
     ap.add_argument("--fd-cells", type=int, default=None,
                     help="adjoint: design cells checked by central differences (32 alone, 4 as a leg)")
+    ap.add_argument("--launch", type=int, default=0,
+                    help="sweep: two-step passes launched per pass (1) or persistently (2); 0 = autotuned")
     ap.add_argument("--graph-chunk", type=int, default=0,
                     help="sweep: steps per captured hipGraph (0 = autotuned, 1 = direct launches)")
     ap.add_argument("--engine", default="fused", choices=["fused", "split"])

@@ -124,7 +124,8 @@ class fg_sim_desc(C.Structure):
 class fg_tune(C.Structure):
     _fields_ = [("tj", u32), ("seg", u32), ("chunk", u32), ("depth", u32), ("cache", C.c_char_p),
                 ("used_tj", u32), ("used_seg", u32), ("used_chunk", u32), ("from_cache", u32),
-                ("used_depth", u32), ("kept_steps", u32), ("tune_ms", f64), ("step_ms", f64)]
+                ("used_depth", u32), ("kept_steps", u32), ("tune_ms", f64), ("step_ms", f64),
+                ("launch", u32), ("used_launch", u32)]
 
 
 class fg_spectra(C.Structure):

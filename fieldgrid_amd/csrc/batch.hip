@@ -50,6 +50,7 @@ struct BatchRun {
     FixArgs fix2;
     WireArgs wa;     // thin wires (wa.w NULL: none): the run's blocks take the wire march
     SheetArgs sheet;  // resistive sheets, corrected after the march (sheet.n 0: none)
+    Cap2 cp;          // two-step runs: step n's port, loop and probe values, captured in the pass
 };
 
 struct BatchBlock {
@@ -141,14 +142,170 @@ __global__ __launch_bounds__(256) void k_observe_pair_batch(const BatchRun *__re
                                                             uint32_t s_off) {
     const BatchRun &r = runs[list[blockIdx.y]];
     observe_pair(r.g, r.obs, r.fix, r.E[par], r.H[par], r.psi[par], r.E[par ^ 1u], r.H[par ^ 1u], s_off, blockIdx.x,
-                 Cap2{});
+                 Cap2{}, threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_observe_batch(const BatchRun *__restrict__ runs,
                                                        const uint32_t *__restrict__ list, uint32_t par,
                                                        uint32_t s_off) {
     const BatchRun &r = runs[list[blockIdx.y]];
-    observe(r.g, r.obs, r.E[par ^ 1u], r.H[par ^ 1u], s_off, blockIdx.x);
+    observe(r.g, r.obs, r.E[par ^ 1u], r.H[par ^ 1u], s_off, blockIdx.x, threadIdx.x);
+}
+
+// ---- persistent two-step passes ----------------------------------------------------------------
+//
+// One launch runs `passes` two-step passes of every run it lists, with no grid-wide boundary
+// between passes: a run's next pass starts as soon as its previous one is done, while other runs
+// are still in theirs. The work items are the runs' march blocks (k_march2_batch's); the same
+// list serves every pass. A run's pass ends with its tail: the second step's sparse fixes
+// (k_fix_batch), then both steps' observers (k_observe_pair_batch, reading step n's port and
+// probe values from the pass's captures). Within one run the order is that of the per-pass
+// launches: all march blocks of pass p, then its tail, then pass p+1's blocks (which overwrite
+// the copy pass p read, and read what its tail wrote). A per-run counter of finished work
+// carries it: each block adds 1 when done, and the block that completes the run's pass runs the
+// tail and adds 1 more, so a block of pass p waits until the counter reaches p * (blocks + 1).
+//
+// The items sit in one queue per XCD (a workgroup takes from queue blockIdx.x % xcds, the XCD
+// the dispatcher places it on), each run's blocks split into contiguous tile ranges over the
+// queues as xcd_order does, so neighbouring tiles share an L2. Every queue is ordered by pass,
+// and a block depends only on blocks (and tails) of earlier passes. With every workgroup
+// resident (a cooperative launch) the launch cannot deadlock: the blocks of the earliest
+// unfinished pass have all they need, and each queue reaches its own ones before any later
+// pass's. A wait that exceeds `limit` ticks of the wall clock sets the abort flag, which every
+// workgroup checks before it takes an item and while it waits, so the launch ends and the host
+// reports the error instead of the GPU hanging.
+//
+// Results are those of the per-pass launches bit for bit: each item runs the same body on the
+// same inputs. Workgroups (possibly on different XCDs, whose L2s are separate) hand data on
+// with agent-scope release fences before a count and acquire fences after a wait.
+
+// Thread 0 takes the workgroup's next item from queue q and waits until its run has finished
+// the passes before the item's. Returns false when the queue is drained or the launch aborted;
+// else the item's position in `items`, its pass and its run.
+__device__ __forceinline__ bool persist_take(const BatchBlock *__restrict__ items, const uint32_t *__restrict__ per_pass,
+                                             uint32_t *next, uint32_t *done, uint32_t *abort, uint32_t base,
+                                             uint32_t qlen, uint32_t passes, uint64_t limit, uint32_t &pos, uint32_t &p,
+                                             uint32_t &run) {
+    if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    const uint32_t idx = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (idx >= qlen * passes) return false;
+    p = idx / qlen;
+    pos = base + idx - p * qlen;
+    run = items[pos].run;
+    const uint32_t need = p * per_pass[run];
+    const uint64_t t0 = uint64_t(wall_clock64());
+    while (__hip_atomic_load(&done[run], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (uint64_t(wall_clock64()) - t0 > limit) {
+            __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// The launch's arguments other than the run table and the item list, in device memory.
+struct PersistArgs {
+    const uint32_t *queues;    // queue x: items[queues[x] .. queues[x] + queues[xcds + x])
+    const uint32_t *per_pass;  // per run: march blocks per pass + 1 (the tail)
+    uint32_t *next;            // per queue: items taken (zero at launch)
+    uint32_t *done;            // per run: blocks and tails finished in this launch (zero at launch)
+    uint32_t *abort;           // nonzero: a wait timed out
+    uint64_t limit;            // wall-clock ticks a wait may last
+    uint32_t passes, par0, seg, xcds, s_base;  // s_base: step offset of the launch's first pass
+};
+
+// A copy of p the compiler cannot see through: loads through it are not hoisted out of a loop.
+template <class T>
+__device__ __forceinline__ const T *fresh(const T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// Nothing but three pointers is held in registers across the march: the run table and the item
+// list (restrict kernel arguments, read with scalar loads as k_march2_batch reads them) and the
+// other arguments' address. Those arguments are re-read where each part of the loop needs them,
+// and what crosses a barrier (the item's position, pass and run, whether the block completed its
+// run's pass) goes through LDS, made uniform by readfirstlane. The march body then keeps
+// k_march2_batch's registers: it sits at the 128-VGPR limit of a 1024-thread block, and values
+// held across it spilled. The tail reads step n's observers from captures for the same reason:
+// recomputing them (observe_mid) next to the march made the march spill.
+template <bool WIDE, bool LDS_SETS>
+__global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_persist(const BatchRun *__restrict__ runs,
+                                                                   const BatchBlock *__restrict__ items,
+                                                                   const PersistArgs *args) {
+    __shared__ March2Lds<TJ2, LDS_SETS, true> lds;
+    __shared__ uint32_t s_go, s_pos, s_p, s_run, s_last;
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    for (;;) {
+        if (lead) {
+            const PersistArgs &A = *fresh(args);
+            const uint32_t q = blockIdx.x % A.xcds;
+            uint32_t pos = 0, p = 0, run = 0;
+            s_go = persist_take(items, A.per_pass, A.next + q, A.done, A.abort, A.queues[q], A.queues[A.xcds + q],
+                                A.passes, A.limit, pos, p, run);
+            s_pos = pos, s_p = p, s_run = run;
+        }
+        __syncthreads();
+        if (!__builtin_amdgcn_readfirstlane(s_go)) break;  // queue drained or aborted
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        {
+            const PersistArgs &A = *fresh(args);
+            const uint32_t p = __builtin_amdgcn_readfirstlane(s_p);
+            const BatchBlock bb = items[__builtin_amdgcn_readfirstlane(s_pos)];
+            const BatchRun &r = runs[bb.run];
+            const uint32_t par = (__builtin_amdgcn_readfirstlane(A.par0) + p) & 1u;
+            const uint32_t s_off = __builtin_amdgcn_readfirstlane(A.s_base) + 2u * p;
+            const uint32_t seg = __builtin_amdgcn_readfirstlane(A.seg);
+            const uint32_t n = r.fix.step[0] + s_off;
+            const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
+            const MarchBlock mb{bb.bx, bb.by, bb.bz & 0x3FFFFFFFu};
+            const int cls = int(bb.bz >> 30);
+#define FG_M2(C)                                                                                                    \
+    march2_ahead<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u],    \
+                                               r.psi[par], r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, r.cp)
+            if (cls == 2) FG_M2(2);
+            else if (cls == 1) FG_M2(1);
+            else FG_M2(0);
+#undef FG_M2
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (lead) {
+            const PersistArgs &A = *fresh(args);
+            const uint32_t run = s_run;
+            const uint32_t before = __hip_atomic_fetch_add(&A.done[run], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = before + 2u == (s_p + 1u) * A.per_pass[run];  // this block completed the run's pass
+        }
+        __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(s_last)) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const PersistArgs &A = *fresh(args);
+            const uint32_t p = __builtin_amdgcn_readfirstlane(s_p), run = __builtin_amdgcn_readfirstlane(s_run);
+            const BatchRun &r = runs[run];
+            const uint32_t par = (__builtin_amdgcn_readfirstlane(A.par0) + p) & 1u;
+            const uint32_t s_off = __builtin_amdgcn_readfirstlane(A.s_base) + 2u * p;
+            const uint32_t flat = threadIdx.y * 64 + threadIdx.x;
+            const FixArgs &f2 = r.fix2;
+            const uint64_t ns = f2.n_src > f2.n_ind ? f2.n_src : f2.n_ind, nf = f2.n_debye > ns ? f2.n_debye : ns;
+            for (uint64_t t = flat; t < nf; t += 64 * (TJ2 + 4))
+                fix_step(r.g, f2, r.E[par ^ 1u], r.E[par], s_off + 1, t);
+            __syncthreads();
+            const uint32_t nob = r.obs.blk_ports + r.obs.blk_probes;  // two-step batches: no plane DFT, NTFF
+            for (uint32_t vb = flat / 256; vb < nob; vb += (64 * (TJ2 + 4)) / 256)
+                observe_pair_ports_cap(r.g, r.obs, r.E[par ^ 1u], r.H[par ^ 1u], s_off, vb, r.cp, flat % 256);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (lead) __hip_atomic_fetch_add(&fresh(args)->done[run], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ void k_set_persist_args(PersistArgs *dst, PersistArgs v) { *dst = v; }
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = 0u;
 }
 
 __global__ void k_batch_set_step(uint32_t *step, uint32_t value) { step[0] = value; }
@@ -225,6 +382,18 @@ struct Batch {
     uint32_t tj = 14, seg = 32, chunk = 32, xcds = 8;
     uint32_t seg_base = 32;  // tuned planes per block; plan() halves it while too few blocks remain
     uint32_t cus = 256;
+    // persistent two-step launches (k_march2_persist): per key (index width x LDS sets) the XCD
+    // queues of one pass, their offsets and lengths (2 x xcds), the runs' items per pass, and the
+    // launch's counters
+    bool persist = false;      // two-step passes go through persistent launches
+    bool persist_ok = false;   // the device takes cooperative launches of the kernel
+    uint32_t persist_grid = 0;
+    uint64_t persist_limit = 0;  // wall-clock ticks a dependency wait may last
+    DevBuf<BatchBlock> pitems[4];
+    DevBuf<uint32_t> pq[4];
+    uint32_t p_qmax[4] = {};   // longest queue of each key (0: the key has no runs)
+    DevBuf<uint32_t> per_pass, pnext, pdone, pabort;
+    DevBuf<PersistArgs> pargs;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> ev;
@@ -243,6 +412,53 @@ struct Batch {
         else if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// The persistent launches' queues for the runs still going (k_march2_persist): per key, XCD
+// queue x holds, for each run in turn, the x-th contiguous range of each of its block classes
+// (x/y-slab blocks, then z-slab, then interior: the longest first). The same queues serve every
+// pass of a launch; per run, its blocks per pass plus one (the tail).
+static void plan_persist(Batch &b, const std::vector<uint8_t> &active) {
+    const uint32_t X = b.xcds;
+    std::vector<uint32_t> pp(b.runs.size(), 0);
+    std::vector<MarchBlock> cls[3];
+    for (int key = 0; key < 4; key++) {
+        std::vector<std::vector<BatchBlock>> mq(X);
+        bool any = false;
+        for (uint32_t r = 0; r < b.runs.size(); r++) {
+            if (!active[r]) continue;
+            const Session &s = *b.runs[r];
+            if ((s.geo.wide ? 2 : 0) + (s.geo.n_sets <= uint32_t(SETS_LDS) ? 1 : 0) != key) continue;
+            march_classes2(s, TJ2, b.seg2, cls);
+            uint32_t m = 0;
+            for (int c = 2; c >= 0; c--) {
+                const size_t N = cls[c].size(), per = (N + X - 1) / X;
+                for (size_t L = 0; L < N; L++) {
+                    const MarchBlock &mb = cls[c][L];
+                    mq[L / per].push_back(BatchBlock{mb.bx, mb.by, mb.bz | (uint32_t(c) << 30), r});
+                }
+                m += uint32_t(N);
+            }
+            pp[r] = m + 1;
+            any = true;
+        }
+        std::vector<BatchBlock> items;
+        std::vector<uint32_t> q(2 * X);
+        uint32_t qmax = 0;
+        for (uint32_t x = 0; x < X; x++) {
+            q[x] = uint32_t(items.size());
+            items.insert(items.end(), mq[x].begin(), mq[x].end());
+            q[X + x] = uint32_t(items.size()) - q[x];
+            qmax = std::max(qmax, q[X + x]);
+        }
+        b.p_qmax[key] = any ? std::max(qmax, 1u) : 0u;
+        if (!any) continue;
+        b.pitems[key].refill(items.data(), items.size(), b.stream);
+        b.pq[key].refill(q.data(), q.size(), b.stream);
+        FG_CHECK(hipStreamSynchronize(b.stream));  // items and q go out of scope
+    }
+    b.per_pass.refill(pp.data(), pp.size(), b.stream);
+    FG_CHECK(hipStreamSynchronize(b.stream));  // the host vectors go out of scope
+}
 
 // Rebuild the merged block lists and the fix/observe run lists for the runs still going.
 static void plan(Batch &b, const std::vector<uint8_t> &active) {
@@ -297,6 +513,7 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
         if (s.fix_blocks) fl.push_back(r), b.fix_gx = std::max(b.fix_gx, s.fix_blocks);
         if (s.obs_blocks) ol.push_back(r), b.obs_gx = std::max(b.obs_gx, s.obs_blocks);
     }
+    if (b.depth == 2 && b.persist) plan_persist(b, active);
     for (int q = 0; q < N_LISTS; q++) {
         xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
         xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
@@ -402,6 +619,44 @@ static void launch_step(Batch &b, uint32_t s_off, uint32_t par) {
                            par, s_off);
 }
 
+template <bool W, bool L>
+static void launch_persist_key(Batch &b, int key, uint32_t passes, uint32_t par, uint32_t s_base) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((uint32_t(b.runs.size()) + 255) / 256), dim3(256), 0, b.stream, b.pdone.p,
+                       uint32_t(b.runs.size()));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, b.stream, b.pnext.p, b.xcds);
+    // the arguments go to device memory in stream order (k_march2_persist reads them there)
+    const PersistArgs pa{b.pq[key].p, b.per_pass.p, b.pnext.p, b.pdone.p, b.pabort.p, b.persist_limit,
+                         passes, par, b.seg2, b.xcds, s_base};
+    hipLaunchKernelGGL(k_set_persist_args, dim3(1), dim3(1), 0, b.stream, b.pargs.p, pa);
+    const BatchRun *runs = b.table.p;
+    const BatchBlock *items = b.pitems[key].p;
+    const PersistArgs *args = b.pargs.p;
+    // every entry is the address of one argument of k_march2_persist, in its order
+    void *kargs[] = {&runs, &items, &args};
+    FG_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_march2_persist<W, L>), dim3(b.persist_grid),
+                                        dim3(64, TJ2 + 4), kargs, 0, b.stream));
+}
+
+// `passes` two-step passes of every run still going from step offset 0, as persistent launches:
+// one per key, each of at most as many passes as keep its item count in 32 bits.
+static void launch_persist(Batch &b, uint64_t passes, uint32_t par) {
+    for (int key = 0; key < 4; key++) {
+        if (!b.p_qmax[key]) continue;
+        const uint64_t most = std::max<uint64_t>(1, std::min<uint64_t>(0xFFFFFFFFull / b.p_qmax[key], 1u << 16));
+        for (uint64_t done = 0; done < passes;) {
+            const uint32_t n = uint32_t(std::min<uint64_t>(most, passes - done));
+            const uint32_t pr = par ^ uint32_t(done & 1u), s_base = uint32_t(2 * done);
+            switch (key) {
+                case 0: launch_persist_key<false, false>(b, key, n, pr, s_base); break;
+                case 1: launch_persist_key<false, true>(b, key, n, pr, s_base); break;
+                case 2: launch_persist_key<true, false>(b, key, n, pr, s_base); break;
+                default: launch_persist_key<true, true>(b, key, n, pr, s_base); break;
+            }
+            done += n;
+        }
+    }
+}
+
 // Two steps (s_off, s_off + 1) of every run still going in one pass of the two-step march.
 static void launch_pair(Batch &b, uint32_t s_off, uint32_t par) {
     launch_march2(b, par, s_off);
@@ -430,6 +685,19 @@ static void capture(Batch &b) {
 // Enqueue `count` steps from step `from`: graph replays while on copy 0 and a whole chunk fits,
 // else direct passes (two steps each at depth 2, a single step for an odd remainder).
 static void advance(Batch &b, uint64_t from, uint64_t count, bool graphs) {
+    if (b.persist && b.depth == 2) {
+        // persistent launches for the pairs, then a single step for an odd remainder
+        hipLaunchKernelGGL(k_batch_set_step, dim3(1), dim3(1), 0, b.stream, b.step.p, uint32_t(from));
+        const uint64_t passes = count / 2;
+        if (passes) launch_persist(b, passes, b.par);
+        b.par ^= uint32_t(passes & 1u);
+        if (count & 1u) {
+            launch_step(b, uint32_t(2 * passes), b.par);
+            b.par ^= 1u;
+        }
+        FG_CHECK(hipGetLastError());
+        return;
+    }
     uint64_t done = 0;
     const bool replay = graphs && b.chunk >= 2;
     while (done < count) {
@@ -488,7 +756,54 @@ static double time_steps(Batch &b, uint64_t &at, uint32_t warm, uint32_t steps, 
     return double(ms) / steps;
 }
 
-// ---- tuning cache: {"<shape key>": [tj, seg, chunk, step_ms], ...} ---------------------------
+// Whether the device takes the persistent launches, and their grid: every workgroup resident at
+// once (a cooperative launch), one per CU slot that all four instances fit, a multiple of the
+// XCD count so each queue has as many workgroups.
+static void persist_setup(Batch &b, uint32_t count) {
+    // Switched off: the first GPU run of these launches (a 5-run batch of small grids, tuned
+    // with persistent launches) did not finish, and its cause is not known. The per-pass
+    // launches serve every batch meanwhile.
+    constexpr bool PERSIST_ENABLED = false;
+    if (!PERSIST_ENABLED) return;
+    int coop = 0, khz = 0;
+    FG_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b.dev->ordinal));
+    FG_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b.dev->ordinal));
+    if (!coop || khz <= 0 || b.xcds == 0) return;
+    int least = 1 << 30;
+    const void *fns[4] = {reinterpret_cast<const void *>(&k_march2_persist<false, false>),
+                          reinterpret_cast<const void *>(&k_march2_persist<false, true>),
+                          reinterpret_cast<const void *>(&k_march2_persist<true, false>),
+                          reinterpret_cast<const void *>(&k_march2_persist<true, true>)};
+    for (const void *f : fns) {
+        int nb = 0;
+        FG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * (TJ2 + 4), 0));
+        least = std::min(least, nb);
+    }
+    const uint32_t grid = uint32_t(std::max(least, 0)) * b.cus / b.xcds * b.xcds;
+    if (!grid) return;
+    b.persist_grid = grid;
+    b.persist_limit = uint64_t(khz) * 2000ull;  // 2 s: a legitimate wait is one block's lifetime
+    b.pnext.alloc(b.xcds);
+    b.pdone.alloc(count);
+    b.pabort.alloc(1);
+    b.pargs.alloc(1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, b.stream, b.pabort.p, 1u);
+    FG_CHECK(hipGetLastError());
+    b.persist_ok = true;
+}
+
+// Fail the batch when a persistent launch gave up on a dependency wait (k_march2_persist).
+static void check_abort(Batch &b) {
+    if (!b.persist_ok) return;
+    uint32_t flag = 0;
+    FG_CHECK(hipMemcpyAsync(&flag, b.pabort.p, 4, hipMemcpyDeviceToHost, b.stream));
+    FG_CHECK(hipStreamSynchronize(b.stream));
+    if (flag)
+        throw HipError{"fg_run_batch: a persistent pass waited too long for its inputs; the launch was abandoned",
+                       FG_HIP};
+}
+
+// ---- tuning cache: {"<shape key>": [tj, seg, chunk, step_ms, depth, persistent], ...} ----------
 
 static std::map<std::string, std::vector<double>> read_cache(const char *path) {
     std::map<std::string, std::vector<double>> m;
@@ -568,7 +883,14 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
     double step_ms = 0.0;
     uint64_t at = 0;  // steps the trial passes have taken
     auto it = cache.find(key);
-    const bool ask = ask_tj || ask_seg || ask_chunk || ask_depth;
+    // persistent launches (two steps per pass only): tuned unless the caller chose
+    if (t && t->launch == 2 && !b.persist_ok)
+        throw HipError{"fg_run_batch: persistent launches are not available (switched off, or the device takes no "
+                       "cooperative launches)",
+                       FG_UNSUPPORTED};
+    const bool ask_launch = (!t || !t->launch) && b.persist_ok;
+    b.persist = t && t->launch == 2 && b.persist_ok;
+    const bool ask = ask_tj || ask_seg || ask_chunk || ask_depth || ask_launch;
     if (ask && it != cache.end()) {
         const std::vector<double> &v = it->second;
         const uint32_t depth = v.size() > 4 && uint32_t(v[4]) == 2 && b.pairs_ok ? 2 : 1;
@@ -579,6 +901,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
             b.seg = b.seg_base = b.seg2 = b.seg2_base = sg;
         }
         if (ask_chunk) b.chunk = std::max<uint32_t>(uint32_t(v[2]), 1);
+        if (ask_launch) b.persist = v.size() > 5 && v[5] != 0.0;
         step_ms = v.size() > 3 ? v[3] : 0.0;
         from_cache = true;
     } else if (ask) {
@@ -630,6 +953,22 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
             }
             b.drop_graph();
         }
+        if (ask_launch && b.depth == 2) {
+            b.chunk = b.chunk > 1 ? (b.chunk + 3) / 4 * 4 : b.chunk;  // whole pairs of passes per replay
+            // persistent launches against the per-pass launches just chosen, over the same
+            // number of steps (a whole graph chunk, at least 16)
+            const uint32_t n = std::max<uint32_t>(16, (b.chunk + 3) / 4 * 4);
+            b.persist = false;
+            plan(b, active);
+            const double per = time_steps(b, at, 4, n, b.chunk >= 2);
+            b.persist = true;
+            plan(b, active);
+            const double pst = time_steps(b, at, 4, n, false);
+            check_abort(b);
+            b.persist = pst < per;
+            step_ms = std::min(per, pst);
+            b.drop_graph();
+        }
         FG_CHECK(hipStreamSynchronize(b.stream));
         if (at >= first_check) {
             for (auto &s : b.runs) reset(*s);
@@ -637,7 +976,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
             at = 0;
         }
         cache[key] = {double(b.tj), double(b.depth == 2 ? b.seg2_base : b.seg_base), double(b.chunk), step_ms,
-                      double(b.depth)};
+                      double(b.depth), b.persist ? 1.0 : 0.0};
         write_cache(t ? t->cache : nullptr, cache);
     }
     // replays start and end on copy 0: an even number of passes per chunk
@@ -650,6 +989,7 @@ static uint64_t autotune(Batch &b, fg_tune *t, std::vector<uint8_t> &active, uin
         t->used_tj = b.depth == 2 ? TJ2 : b.tj, t->used_seg = b.depth == 2 ? b.seg2_base : b.seg_base;
         t->used_chunk = b.chunk, t->from_cache = from_cache ? 1 : 0, t->used_depth = b.depth;
         t->step_ms = step_ms;
+        t->used_launch = b.persist && b.depth == 2 ? 2u : 1u;
         t->kept_steps = uint32_t(at);
         t->tune_ms = tuned ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
     }
@@ -706,7 +1046,7 @@ static void make_sessions(Batch &b, uint32_t count, const fg_problem *const *pro
         try {
             FG_CHECK(hipSetDevice(b.dev->ordinal));
             for (uint32_t r; (r = next.fetch_add(1)) < count;)
-                b.runs[r].reset(session_new(b.dev, problems[r], driven[r], &pulses[r], &xs[r]));
+                b.runs[r].reset(session_new(b.dev, problems[r], driven[r], &pulses[r], &xs[r], true));
         } catch (...) {
             std::lock_guard<std::mutex> lock(m);
             if (!first) first = std::current_exception();
@@ -776,7 +1116,11 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             t.db = Deb2{s.deb2_row.p, s.deb2_map.p, s.deb2_jn.p, s.deb2_eo.p};
             t.fix2 = t.fix;
             if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
-            if (!s.fuse2 || s.cap_start.n) all_pairs = false;  // the batch hands observers no captures
+            // the per-pass launches hand observers no captures: no plane DFT or NTFF in a pair
+            if (!s.fuse2 || s.obs.plane_n || s.obs.n_patches) all_pairs = false;
+            if (s.fuse2)
+                t.cp = Cap2{s.cap_start.p, s.cap_seg.p, s.cap_buf.p, s.cap_ntff.p, s.cap_plane.p, s.cap_obs.p,
+                            s.cap_loop0, s.cap_probe0};
             b.seg2_cap = std::min(b.seg2_cap, march2_seg_cap(march2_plane_bytes(s.n, s.pml)));
             t.wa = WireArgs{s.wire_start.p, s.wire_ent.p, s.wires.n ? s.wires.p : nullptr};
             t.sheet = s.sheet;
@@ -784,6 +1128,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
         }
         b.pairs_ok = all_pairs && count > 0;
         b.table.upload(table.data(), table.size(), b.stream);
+        if (b.pairs_ok) persist_setup(b, count);
         std::vector<uint8_t> active(count, 0);
         for (uint32_t r = 0; r < count; r++) active[r] = extras[r]->max_steps > 0;
         // per-run decay checks at multiples of check_every and at max_steps (run.rs:681-748)
@@ -812,6 +1157,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             advance(b, global, next - global, graphs);
             global = next;
             FG_CHECK(hipStreamSynchronize(b.stream));
+            check_abort(b);
             double least = INFINITY;
             for (uint32_t r = 0; r < count; r++) {
                 if (!active[r]) continue;

@@ -775,15 +775,15 @@ __device__ __forceinline__ void ntff_acc(const ObsArgs &a, uint64_t gi, uint32_t
     }
 }
 
-// Block `blk` of one run's observers (cpu.rs:309-415): port V/I waves, then point probes,
-// then plane DFT, then NTFF patches, each section a range of 256-thread blocks.
-__device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const float *__restrict__ E,
-                                        const float *__restrict__ H, uint32_t s_off, uint32_t blk) {
+// The port V/I and probe blocks of observe() (blk < a.blk_ports + a.blk_probes).
+__device__ __forceinline__ void observe_ports(const Geo &g, const ObsArgs &a, const float *__restrict__ E,
+                                              const float *__restrict__ H, uint32_t s_off, uint32_t blk,
+                                              uint32_t tid) {
     const uint32_t n = a.step[0] + s_off;
     const uint64_t nn = g.nn;
     if (blk < a.blk_ports) {
         // one wave per port; f32 products summed in f64 like cpu.rs:317-328
-        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+        const uint32_t p = blk * 4 + tid / 64, lane = tid % 64;
         if (p >= a.n_ports) return;
         const fg_port d = a.ports[p];
         double v = 0.0, cur = 0.0;
@@ -805,16 +805,28 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
     }
     blk -= a.blk_ports;
     if (blk < a.blk_probes) {
-        const uint32_t q = blk * 256 + threadIdx.x;
+        const uint32_t q = blk * 256 + tid;
         if (q < a.n_probes && n < a.cap) {
             const uint64_t f = a.probe_f[q];
             a.probes[uint64_t(n) * a.n_probes + q] = f < 3 * nn ? E[f] : H[f - 3 * nn];
         }
+    }
+}
+
+// Block `blk` of one run's observers (cpu.rs:309-415): port V/I waves, then point probes,
+// then plane DFT, then NTFF patches, each section a range of 256-thread blocks; `tid` is the
+// thread's index in its 256-thread block (its four waves whole).
+__device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const float *__restrict__ E,
+                                        const float *__restrict__ H, uint32_t s_off, uint32_t blk, uint32_t tid) {
+    if (blk < a.blk_ports + a.blk_probes) {
+        observe_ports(g, a, E, H, s_off, blk, tid);
         return;
     }
-    blk -= a.blk_probes;
+    const uint32_t n = a.step[0] + s_off;
+    const uint64_t nn = g.nn;
+    blk -= a.blk_ports + a.blk_probes;
     if (blk < a.blk_plane) {
-        const uint32_t q = blk * 256 + threadIdx.x;
+        const uint32_t q = blk * 256 + tid;
         if (q >= a.plane_n) return;
         const uint32_t i = q / g.n1, j = q % g.n1;
         const uint64_t id = (uint64_t(i) * g.n1 + j) * g.n2 + a.plane_k;
@@ -824,7 +836,7 @@ __device__ __forceinline__ void observe(const Geo &g, const ObsArgs &a, const fl
     }
     blk -= a.blk_plane;
     if (blk < a.blk_ntff) {
-        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        const uint64_t gi = uint64_t(blk) * 256 + tid;
         if (gi >= a.n_patches) return;
         const uint32_t *ix = a.patch + gi * 13;
         const uint32_t meta = ix[0], ax = meta & 3u, u = (ax + 1) % 3, v = (ax + 2) % 3;
@@ -881,6 +893,11 @@ struct Cap2 {
     float *buf;
     const uint32_t *ntff;
     const uint32_t *plane;
+    // with observer captures (NULL: none): the buffer index of every port edge's E (in the
+    // order of ObsArgs::edges), then from obs_loop0 of every loop term's H (ObsArgs::loops),
+    // then from obs_probe0 of every probe's field (ObsArgs::probe_f)
+    const uint32_t *obs;
+    uint32_t obs_loop0, obs_probe0;
 };
 
 // march2's LDS, declared once by the kernel that calls it, so that a kernel running blocks of
@@ -1923,12 +1940,12 @@ __device__ inline float field_mid(const Geo &g, const FixArgs &f, float pl, cons
 // field_mid above), block `blk` of the observer launch as in observe().
 __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs &f, const float *__restrict__ E,
                                    const float *__restrict__ H, const float *__restrict__ psi, uint32_t s_off,
-                                   uint32_t blk, const Cap2 &cp) {
+                                   uint32_t blk, const Cap2 &cp, uint32_t tid) {
     const uint32_t n = a.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
     const uint64_t nn = g.nn;
     if (blk < a.blk_ports) {  // as observe(): one wave per port, f32 products summed in f64
-        const uint32_t p = blk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+        const uint32_t p = blk * 4 + tid / 64, lane = tid % 64;
         if (p >= a.n_ports) return;
         const fg_port d = a.ports[p];
         double v = 0.0, cur = 0.0;
@@ -1950,14 +1967,14 @@ __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs
     }
     blk -= a.blk_ports;
     if (blk < a.blk_probes) {
-        const uint32_t q = blk * 256 + threadIdx.x;
+        const uint32_t q = blk * 256 + tid;
         if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = field_mid(g, f, pl, E, H, psi, a.probe_f[q]);
         return;
     }
     // plane DFT and NTFF from the values the pass captured (Cap2)
     blk -= a.blk_probes;
     if (blk < a.blk_plane) {
-        const uint32_t q = blk * 256 + threadIdx.x;
+        const uint32_t q = blk * 256 + tid;
         if (q >= a.plane_n) return;
         const uint32_t *ix = cp.plane + uint64_t(q) * 5;
         const float fv[5] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]], cp.buf[ix[4]]};
@@ -1966,13 +1983,46 @@ __device__ inline void observe_mid(const Geo &g, const ObsArgs &a, const FixArgs
     }
     blk -= a.blk_plane;
     if (blk < a.blk_ntff) {
-        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        const uint64_t gi = uint64_t(blk) * 256 + tid;
         if (gi >= a.n_patches) return;
         const uint32_t *ix = cp.ntff + gi * 12;
         const float e[4] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]]};
         const float h[8] = {cp.buf[ix[4]], cp.buf[ix[5]], cp.buf[ix[6]], cp.buf[ix[7]],
                             cp.buf[ix[8]], cp.buf[ix[9]], cp.buf[ix[10]], cp.buf[ix[11]]};
         ntff_acc(a, gi, n, a.patch[gi * 13], e, h);
+    }
+}
+
+// observe_mid's port V/I and probes from the values the pass captured at those points (Cap2::obs)
+// instead of recomputing them: the same values (the capture is step n's final field, as the
+// recomputation gives it) summed in the same order, so the series are bit-identical.
+__device__ inline void observe_mid_cap(const ObsArgs &a, uint32_t s_off, uint32_t blk, const Cap2 &cp, uint32_t tid) {
+    const uint32_t n = a.step[0] + s_off;
+    if (blk < a.blk_ports) {
+        const uint32_t p = blk * 4 + tid / 64, lane = tid % 64;
+        if (p >= a.n_ports) return;
+        const fg_port d = a.ports[p];
+        double v = 0.0, cur = 0.0;
+        for (uint32_t q = lane; q < d.n_edges; q += 64) {
+            const fg_port_edge pe = a.edges[d.first_edge + q];
+            v -= double(cp.buf[cp.obs[d.first_edge + q]] * pe.w);
+        }
+        for (uint32_t q = lane; q < d.n_loops; q += 64) {
+            const fg_loop_term lt = a.loops[d.first_loop + q];
+            cur += double(cp.buf[cp.obs[cp.obs_loop0 + d.first_loop + q]] * lt.w);
+        }
+        v = wave_sum(v);
+        cur = wave_sum(cur);
+        if (lane == 0 && n < a.cap) {
+            a.series[(uint64_t(n) * a.n_ports + p) * 2] = float(v);
+            a.series[(uint64_t(n) * a.n_ports + p) * 2 + 1] = float(cur);
+        }
+        return;
+    }
+    blk -= a.blk_ports;
+    if (blk < a.blk_probes) {
+        const uint32_t q = blk * 256 + tid;
+        if (q < a.n_probes && n < a.cap) a.probes[uint64_t(n) * a.n_probes + q] = cp.buf[cp.obs[cp.obs_probe0 + q]];
     }
 }
 
@@ -2055,20 +2105,41 @@ __device__ __forceinline__ void ntff_acc2(const ObsArgs &a, uint64_t gi, uint32_
     }
 }
 
+// The plane-DFT and NTFF blocks of observe_pair (blk counted from the first plane block).
+__device__ __forceinline__ void observe_pair_fields(const Geo &g, const ObsArgs &a, const float *__restrict__ En,
+                                                    const float *__restrict__ Hn, uint32_t s_off, uint32_t blk,
+                                                    const Cap2 &cp, uint32_t tid);
+
 __device__ inline void observe_pair(const Geo &g, const ObsArgs &a, const FixArgs &f, const float *__restrict__ Eo,
                                     const float *__restrict__ Ho, const float *__restrict__ psio,
                                     const float *__restrict__ En, const float *__restrict__ Hn, uint32_t s_off,
-                                    uint32_t blk, const Cap2 &cp) {
+                                    uint32_t blk, const Cap2 &cp, uint32_t tid) {
     if (blk < a.blk_ports + a.blk_probes) {
-        observe_mid(g, a, f, Eo, Ho, psio, s_off, blk, cp);
-        observe(g, a, En, Hn, s_off + 1, blk);
+        observe_mid(g, a, f, Eo, Ho, psio, s_off, blk, cp, tid);
+        observe(g, a, En, Hn, s_off + 1, blk, tid);
         return;
     }
+    observe_pair_fields(g, a, En, Hn, s_off, blk - (a.blk_ports + a.blk_probes), cp, tid);
+}
+
+// Both steps' port V/I and probes of a two-step pass with no plane DFT or NTFF, step n's from
+// the pass's captures (observe_mid_cap): no recomputation and no field-DFT code, so a kernel
+// that runs it next to the march keeps the march's registers.
+__device__ inline void observe_pair_ports_cap(const Geo &g, const ObsArgs &a, const float *__restrict__ En,
+                                              const float *__restrict__ Hn, uint32_t s_off, uint32_t blk,
+                                              const Cap2 &cp, uint32_t tid) {
+    if (blk >= a.blk_ports + a.blk_probes) return;
+    observe_mid_cap(a, s_off, blk, cp, tid);
+    observe_ports(g, a, En, Hn, s_off + 1, blk, tid);
+}
+
+__device__ __forceinline__ void observe_pair_fields(const Geo &g, const ObsArgs &a, const float *__restrict__ En,
+                                                    const float *__restrict__ Hn, uint32_t s_off, uint32_t blk,
+                                                    const Cap2 &cp, uint32_t tid) {
     const uint32_t n = a.step[0] + s_off;
     const uint64_t nn = g.nn;
-    blk -= a.blk_ports + a.blk_probes;
     if (blk < a.blk_plane) {
-        const uint32_t q = blk * 256 + threadIdx.x;
+        const uint32_t q = blk * 256 + tid;
         if (q >= a.plane_n) return;
         const uint32_t *ix = cp.plane + uint64_t(q) * 5;
         const float f0[5] = {cp.buf[ix[0]], cp.buf[ix[1]], cp.buf[ix[2]], cp.buf[ix[3]], cp.buf[ix[4]]};
@@ -2080,7 +2151,7 @@ __device__ inline void observe_pair(const Geo &g, const ObsArgs &a, const FixArg
     }
     blk -= a.blk_plane;
     if (blk < a.blk_ntff) {
-        const uint64_t gi = uint64_t(blk) * 256 + threadIdx.x;
+        const uint64_t gi = uint64_t(blk) * 256 + tid;
         if (gi >= a.n_patches) return;
         const uint32_t *cx = cp.ntff + gi * 12;
         const float e0[4] = {cp.buf[cx[0]], cp.buf[cx[1]], cp.buf[cx[2]], cp.buf[cx[3]]};

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_fix(Geo g, FixArgs a, float *__restrict
 
 __global__ __launch_bounds__(256) void k_observe(Geo g, ObsArgs a, const float *__restrict__ E,
                                                  const float *__restrict__ H, uint32_t s_off) {
-    observe(g, a, E, H, s_off, blockIdx.x);
+    observe(g, a, E, H, s_off, blockIdx.x, threadIdx.x);
 }
 
 // Two steps of the march per launch (see march2): the first step's pulse value comes from the
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_observe_pair(Geo g, ObsArgs a, FixArgs 
                                                       const float *__restrict__ Ho, const float *__restrict__ psio,
                                                       const float *__restrict__ En, const float *__restrict__ Hn,
                                                       uint32_t s_off, Cap2 cp) {
-    observe_pair(g, a, f, Eo, Ho, psio, En, Hn, s_off, blockIdx.x, cp);
+    observe_pair(g, a, f, Eo, Ho, psio, En, Hn, s_off, blockIdx.x, cp, threadIdx.x);
 }
 
 // The largest coefficient-set index of the per-edge index table (u32 words: one index when
@@ -561,8 +561,9 @@ static void plan_march(Session &s) {
 
 // The points the first step of a two-step pass must hand to its plane-DFT and NTFF observers
 // (Cap2): every (field, node) they read, grouped per (i, j) row into segments of consecutive k
-// with the same fields, and the per-observer index tables into the capture buffer.
-static void capture_points(Session &s, const fg_problem *P, const fg_extras *X) {
+// with the same fields, and the per-observer index tables into the capture buffer. With `obs`,
+// also every port edge, loop term and probe (Cap2::obs).
+static void capture_points(Session &s, const fg_problem *P, const fg_extras *X, bool obs) {
     hipStream_t st = s.stream;
     const uint64_t nn = s.nn, n1 = P->n[1], n2 = P->n[2];
     const ObsArgs &o = s.obs;
@@ -579,8 +580,32 @@ static void capture_points(Session &s, const fg_problem *P, const fg_extras *X) 
         const uint64_t f[12] = {u, u, v, v, 3 + u, 3 + u, 3 + u, 3 + u, 3 + v, 3 + v, 3 + v, 3 + v};
         for (int t = 0; t < 12; t++) nkeys.push_back(f[t] * nn + ix[1 + t]);
     }
+    // port edges, loop terms, probes, at their index in Cap2::obs (P->port_edges, then P->loops,
+    // then X->probes); an entry no port refers to stays unset (~0) and is never read
+    const uint64_t unset = ~0ull;
+    std::vector<uint64_t> okeys;
+    if (obs) {
+        okeys.assign(uint64_t(P->n_port_edges) + P->n_loops + X->n_probes, unset);
+        s.cap_loop0 = P->n_port_edges;
+        s.cap_probe0 = P->n_port_edges + P->n_loops;
+        for (uint32_t p = 0; p < P->n_ports; p++) {
+            const fg_port &d = P->ports[p];
+            for (uint32_t q = 0; q < d.n_edges; q++)
+                okeys[d.first_edge + q] = uint64_t(d.comp) * nn + P->port_edges[d.first_edge + q].id;
+            for (uint32_t q = 0; q < d.n_loops; q++) {
+                const fg_loop_term &lt = P->loops[d.first_loop + q];
+                okeys[s.cap_loop0 + d.first_loop + q] = (3ull + lt.comp) * nn + lt.id;
+            }
+        }
+        for (uint32_t q = 0; q < X->n_probes; q++) {
+            const fg_point_probe &pr = X->probes[q];
+            okeys[s.cap_probe0 + q] = uint64_t(pr.field) * nn + (uint64_t(pr.i) * n1 + pr.j) * n2 + pr.k;
+        }
+    }
     pts.insert(pts.end(), pkeys.begin(), pkeys.end());
     pts.insert(pts.end(), nkeys.begin(), nkeys.end());
+    for (uint64_t key : okeys)
+        if (key != unset) pts.push_back(key);
     // per node id: the mask of captured fields
     std::vector<std::pair<uint64_t, uint32_t>> nodes;  // (id, mask)
     nodes.reserve(pts.size());
@@ -618,9 +643,14 @@ static void capture_points(Session &s, const fg_problem *P, const fg_extras *X) 
         const uint32_t mask = merged[q].second;
         return uint32_t(first[q] + uint32_t(__builtin_popcount(mask & ((1u << c) - 1u))));
     };
-    std::vector<uint32_t> pidx(pkeys.size()), nidx(nkeys.size());
+    std::vector<uint32_t> pidx(pkeys.size()), nidx(nkeys.size()), oidx(okeys.size());
     for (size_t q = 0; q < pkeys.size(); q++) pidx[q] = index(pkeys[q]);
     for (size_t q = 0; q < nkeys.size(); q++) nidx[q] = index(nkeys[q]);
+    for (size_t q = 0; q < okeys.size(); q++) oidx[q] = okeys[q] == unset ? 0u : index(okeys[q]);
+    if (obs) {
+        if (oidx.empty()) s.cap_obs.alloc(1);
+        else s.cap_obs.upload(oidx.data(), oidx.size(), st);
+    }
     s.cap_start.upload(start.data(), start.size(), st);
     s.cap_seg.upload(seg.data(), seg.size(), st);
     s.cap_buf.alloc(std::max<uint64_t>(count, 1));
@@ -632,7 +662,7 @@ static void capture_points(Session &s, const fg_problem *P, const fg_extras *X) 
 }
 
 Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse,
-                            const fg_extras *X) {
+                            const fg_extras *X, bool obs_capture) {
     validate(P, driven, X);
     auto S = std::make_unique<Session>();
     Session &s = *S;
@@ -1011,7 +1041,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
             s.deb2_jn.alloc(std::max<uint64_t>(uint64_t(P->n_poles) * P->n_debye, 1));
             s.deb2_eo.alloc(P->n_debye);
         }
-        if (o.plane_n || o.n_patches) capture_points(s, P, X);
+        if (o.plane_n || o.n_patches || obs_capture) capture_points(s, P, X, obs_capture);
         plan_march2(s);
     }
     FG_CHECK(hipStreamSynchronize(st));

@@ -231,6 +231,8 @@ struct Session {
     DevBuf<int32_t> deb2_row, deb2_map;  // Debye k maps of the two-step march (Deb2)
     DevBuf<float> deb2_jn, deb2_eo;
     DevBuf<uint32_t> cap_start, cap_ntff, cap_plane;  // observed points of the two-step march (Cap2)
+    DevBuf<uint32_t> cap_obs;  // observer captures (Cap2::obs): port edges, loop terms, probes
+    uint32_t cap_loop0 = 0, cap_probe0 = 0;
     DevBuf<uint4> cap_seg;
     DevBuf<float> cap_buf;
     // resistive sheets (records restored from sheets0 on reset) and thin wires
@@ -285,7 +287,10 @@ double session_bytes(const fg_problem *P, bool fused);
 // the session's other device memory (Debye, records, observers, tables), bounded from above
 double session_extra_bytes(const fg_problem *P, const fg_extras *X);
 // One run's device state: validated, allocated and uploaded, fields at zero (engine.hip).
-Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse, const fg_extras *X);
+// obs_capture: where the run takes two steps per pass, its port edges, loop terms and probes are
+// captured too (Cap2::obs), for launches that read step n's observers from captures.
+Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const fg_pulse *pulse, const fg_extras *X,
+                     bool obs_capture = false);
 // Zero every field, psi, Debye, inductor-current and accumulator buffer and the step count on
 // the session's stream, so the run starts over exactly as from session_new.
 void reset(Session &s);

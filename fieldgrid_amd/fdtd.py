@@ -557,13 +557,16 @@ class BatchRun:
 
 @dataclass
 class Tune:
-    """March tile, steps per march pass (1 or 2) and graph chunk for a batch (0 = autotune),
-    with an optional JSON cache. `used` comes back as (tj, seg, chunk, depth)."""
+    """March tile, steps per march pass (1 or 2), graph chunk and launch mode (1 per-pass
+    launches, 2 persistent launches of many passes) for a batch (0 = autotune), with an optional
+    JSON cache. `used` comes back as (tj, seg, chunk, depth) and `used_launch` as 1 or 2."""
     tj: int = 0
     seg: int = 0
     chunk: int = 0
     depth: int = 0
     cache: Optional[str] = None
+    launch: int = 0
+    used_launch: int = 0
     used: Tuple[int, int, int, int] = (0, 0, 0, 0)
     from_cache: bool = False
     tune_ms: float = 0.0
@@ -612,7 +615,7 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
             spec_bufs.append(None)
     t = abi.fg_tune()
     if tune is not None:
-        t.tj, t.seg, t.chunk, t.depth = tune.tj, tune.seg, tune.chunk, tune.depth
+        t.tj, t.seg, t.chunk, t.depth, t.launch = tune.tj, tune.seg, tune.chunk, tune.depth, tune.launch
         t.cache = tune.cache.encode() if tune.cache else None
     errors = []
 
@@ -633,6 +636,7 @@ def run_batch(gpu: Gpu, runs: Sequence[BatchRun], tune: Optional[Tune] = None,
         tune.from_cache = bool(t.from_cache)
         tune.tune_ms, tune.step_ms = t.tune_ms, t.step_ms
         tune.kept_steps = int(t.kept_steps)
+        tune.used_launch = int(t.used_launch)
     records = [_finish(recs[r], bufs_all[r], runs[r].sim, runs[r].extras) for r in range(count)]
     spectra = []
     for r, sb in enumerate(spec_bufs):

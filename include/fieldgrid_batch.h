@@ -44,6 +44,12 @@ typedef struct fg_tune {
     double tune_ms;       /* time spent tuning (0 when the cache or the caller decided),
                              including the kept trial steps                                    */
     double step_ms;       /* mean time of one step of the whole batch, as the tuner measured it */
+    uint32_t launch;      /* in: how two-step passes are launched: 1 = per pass (hipGraph chunks
+                             or direct launches, as chunk says); 2 = persistent: one launch runs
+                             many passes, each run going on to its next pass as soon as its own
+                             previous one is done; 0 = tune (2 only where the device takes
+                             cooperative launches)                                             */
+    uint32_t used_launch; /* out: the launch mode used (1 or 2)                                */
 } fg_tune;
 
 /* Port V/I spectra of one run (crates/sims-antenna/src/fdtd/mod.rs:231-246, per port):

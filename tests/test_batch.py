@@ -14,6 +14,7 @@ import os
 import numpy as np
 import pytest
 
+from fieldgrid_amd import abi
 from fieldgrid_amd import fdtd as F
 from fieldgrid_amd import workloads as W
 from oracle import oracle as O
@@ -75,6 +76,13 @@ def test_the_sweep_recipe_keeps_its_ranges():
         assert 50 <= r.extras.max_steps <= 90 and len(r.freqs) == 5
 
 
+# The persistent launches (fg_tune.launch = 2) are switched off: their first GPU run (the
+# depth2-seg16-persistent case below) did not finish within its time limit, and the cause is
+# not known. Kept, skipped, until it is.
+PERSIST_OFF = pytest.mark.skip(reason="persistent launches switched off: their first GPU run did not finish, cause "
+                                      "unknown")
+
+
 def small_sweep(count=5, steps=(60, 150), **kw):
     return W.sweep_runs(count=count, lo=24, hi=46, steps=steps, nboxes=3, ndebye=1, nfreq=6, **kw)
 
@@ -91,9 +99,11 @@ def _same(a, b, what):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tune", [F.Tune(tj=14, seg=16, chunk=8, depth=1), F.Tune(tj=8, seg=8, chunk=1, depth=1),
-                                  F.Tune(seg=16, chunk=8, depth=2), F.Tune(seg=3, chunk=1, depth=2), F.Tune()],
+                                  F.Tune(seg=16, chunk=8, depth=2, launch=1), F.Tune(seg=3, chunk=1, depth=2, launch=1),
+                                  pytest.param(F.Tune(seg=16, depth=2, launch=2), marks=PERSIST_OFF),
+                                  pytest.param(F.Tune(seg=3, depth=2, launch=2), marks=PERSIST_OFF), F.Tune()],
                          ids=["tj14-seg16-graph8", "tj8-seg8-direct", "depth2-seg16-graph8", "depth2-seg3-direct",
-                              "autotuned"])
+                              "depth2-seg16-persistent", "depth2-seg3-persistent", "autotuned"])
 def test_batched_runs_equal_single_runs(gpu, tune):
     """Every batched record bit-identical to a single fg_run of the same problem, with one step
     per march pass and with two (the runs have Debye boxes: the first step's ADE runs inside the
@@ -106,6 +116,8 @@ def test_batched_runs_equal_single_runs(gpu, tune):
     print(f"  tuned: tj, seg, chunk, depth = {tune.used}, tune {tune.tune_ms:.1f} ms, step {tune.step_ms:.3f} ms")
     if tune.depth:
         assert tune.used[3] == tune.depth
+    if tune.launch:
+        assert tune.used_launch == tune.launch
     if not (tune.tj or tune.seg or tune.chunk or tune.depth):
         assert tune.kept_steps == 0  # a decay check at step 40 falls inside the trials: rolled back
     for q, r in enumerate(runs):
@@ -241,3 +253,37 @@ def test_sheets_and_wires_in_a_batch_equal_single_runs(gpu, tune):
     for q, r in enumerate(runs):
         single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
         _same(recs[q], single, f"run {q}")
+
+
+@PERSIST_OFF
+@pytest.mark.gpu
+@pytest.mark.parametrize("seg", [32, 5])
+def test_persistent_passes_with_probes_equal_single_runs(gpu, seg):
+    """Persistent launches (each run on to its next pass as soon as its previous one is done;
+    step n's port V/I and probes read from the pass's captures) against single runs, bit for
+    bit, with point probes of E and H in every run and odd step counts (a single step ends
+    each window)."""
+    runs = small_sweep(count=6, steps=(41, 97), seed=11)
+    rng = np.random.default_rng(3)
+    for r in runs:
+        n = r.sim.dims()
+        r.extras.check_every = 30
+        # at i = pml + 1, clear of the dielectric boxes (they start at pml + 2): no Debye edge
+        r.extras.probes = [(int(f), 9, int(rng.integers(2, n[1] - 2)), int(rng.integers(2, n[2] - 2)))
+                           for f in (0, 2, 3, 5)]
+    tune = F.Tune(seg=seg, depth=2, launch=2)
+    recs, _ = F.run_batch(gpu, runs, tune=tune, fields=True)
+    assert tune.used[3] == 2 and tune.used_launch == 2
+    for q, r in enumerate(runs):
+        single = F.run(gpu, r.sim, r.driven, r.pulse, r.extras, fields=True)
+        assert np.count_nonzero(single.probes) > 0
+        _same(recs[q], single, f"run {q}")
+
+
+@pytest.mark.gpu
+def test_a_persistent_launch_request_is_refused_while_switched_off(gpu):
+    """fg_tune.launch = 2 is refused with FG_UNSUPPORTED while the persistent launches are off."""
+    runs = small_sweep(count=2, steps=(8, 12))
+    with pytest.raises(abi.FieldgridError) as e:
+        F.run_batch(gpu, runs, tune=F.Tune(seg=16, depth=2, launch=2))
+    assert "persistent launches are not available" in str(e.value)
