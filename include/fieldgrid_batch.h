@@ -47,8 +47,8 @@ typedef struct fg_tune {
     uint32_t launch;      /* in: how two-step passes are launched: 1 = per pass (hipGraph chunks
                              or direct launches, as chunk says); 2 = persistent: one launch runs
                              many passes, each run going on to its next pass as soon as its own
-                             previous one is done; 0 = tune (2 only where the device takes
-                             cooperative launches)                                             */
+                             previous one is done (switched off in this build: refused with
+                             FG_UNSUPPORTED); 0 = tune                                          */
     uint32_t used_launch; /* out: the launch mode used (1 or 2)                                */
 } fg_tune;
 
