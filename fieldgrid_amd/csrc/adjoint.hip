@@ -1366,6 +1366,9 @@ static void adj_march(Session &s, hipStream_t st, const float *Lo, float *Ln, co
 #undef FG_AM2
 }
 
+// Rows per block of the two-step adjoint's x/y-slab class (the other classes take 12).
+constexpr uint32_t ADJ_TJ_XY = 6;
+
 // Two adjoint steps through k_adj_march2 over the two-step adjoint block lists (march2's tile,
 // reading one plane further along i), one launch per CPML class on the stream.
 static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln, const float *Mo, float *Mn,
@@ -1378,6 +1381,16 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
         const uint32_t sg = e ? uint32_t(std::max(1, std::min(atoi(e), SEG_MAX))) : s.seg2;
         std::vector<MarchBlock> cls[3];
         march_classes2(s, 12, sg, cls, 1);
+        // The x/y-slab class runs 6-row tiles (each 12-row tile as two): at 12 rows its 1024
+        // threads leave 128 registers a lane and it spilled 45; 640 threads leave 170 and it
+        // holds 160. Every node's terms are the same whatever the tile, so the results are too.
+        {
+            std::vector<MarchBlock> half;
+            for (const MarchBlock &m : cls[2])
+                for (uint32_t h = 0; h < 2; h++)
+                    if ((2 * m.by + h) * ADJ_TJ_XY < s.n[1]) half.push_back(MarchBlock{m.bx, 2 * m.by + h, m.bz});
+            cls[2].swap(half);
+        }
         for (int c = 0; c < 3; c++) {
             xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
             if (cls[c].empty()) A.ablk2[c].alloc(1);
@@ -1387,7 +1400,7 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
         FG_CHECK(hipStreamSynchronize(s.stream));
         A.adj_seg = sg;
     }
-    const dim3 block(64, 12 + 4);
+    const dim3 block(64, 12 + 4), block_xy(64, ADJ_TJ_XY + 4);
     const uint32_t n0 = uint32_t(A.ablk2[0].n), n1 = uint32_t(A.ablk2[1].n), n2 = uint32_t(A.ablk2[2].n);
     const MarchBlock *b0 = A.ablk2[0].p, *b1 = A.ablk2[1].p, *b2 = A.ablk2[2].p;
     const uint32_t sg = A.adj_seg;
@@ -1397,8 +1410,8 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
                                    sg, ga, tapeB, in, dsB);                                                      \
         if (n1) hipLaunchKernelGGL((k_adj_march2<12, 1, W, L>), dim3(n1), block, 0, st, g, b1, Lo, Ln, Mo, Mn, po, pn, \
                                    sg, ga, tapeB, in, dsB);                                                      \
-        if (n2) hipLaunchKernelGGL((k_adj_march2<12, 2, W, L>), dim3(n2), block, 0, st, g, b2, Lo, Ln, Mo, Mn, po, pn, \
-                                   sg, ga, tapeB, in, dsB);                                                      \
+        if (n2) hipLaunchKernelGGL((k_adj_march2<ADJ_TJ_XY, 2, W, L>), dim3(n2), block_xy, 0, st, g, b2, Lo, Ln, Mo, \
+                                   Mn, po, pn, sg, ga, tapeB, in, dsB);                                          \
     } while (0)
     const bool lds = g.n_sets <= uint32_t(SETS_LDS);
     if (g.wide) {
@@ -1459,12 +1472,14 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     // the two gather kernels; the march ping-pongs lamE (lamE/lamA) and lamH (lamH/lamH2)
     const char *am = getenv("FIELDGRID_ADJ_MARCH");
     const bool march = s.fused && (!(am && am[0] == '0') || media);  // the gather kernels have no media
-    // two adjoint steps per pass (k_adj_march2) only where FIELDGRID_ADJ_PAIRS=1 asks: on the
-    // 192x192x112 design grid it halves the passes but costs 235 us per step against 150 us for
-    // one step per pass (the step is issue-bound, not bandwidth-bound: the two-step tile computes
-    // 1.15x more per owned node-step, and its x/y-slab class spills 45 VGPRs)
+    // two adjoint steps per pass (k_adj_march2) unless FIELDGRID_ADJ_PAIRS=0 asks for one: on the
+    // 192x192x112 design grid the backward takes 589 ms against 628 ms with one step per pass,
+    // its recompute overlapped either way (the step is latency-bound, not bandwidth-bound: the
+    // two-step tile computes more per owned node-step and each class is about one round of
+    // blocks, so halving the passes gains little); runs with Debye media, inductors or thin
+    // wires take one step per pass (their transposes sit between the steps)
     const char *ap = getenv("FIELDGRID_ADJ_PAIRS");
-    const bool pairs = march && !media && ap && ap[0] == '1';
+    const bool pairs = march && !media && !(ap && ap[0] == '0');
     float *LE[2] = {A.lamE.p, A.lamA.p}, *LH[2] = {A.lamH.p, A.lamH2.p};
     int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
