@@ -250,7 +250,8 @@ def adjoint_leg(args, fd_cells):
            "finite_differences": fd, "build_s": round(build_s, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": round(alg),
-                        "kernel": "k_adj_march (one adjoint step per launch; HIP-event busy time of its stream per step)",
+                        "kernel": ("k_adj_march2 (two adjoint steps per launch, per CPML class; HIP-event busy time of "
+                                   "its stream per step, the recompute overlapped on a second stream)"),
                         "step_ms": round(adj_step_ms, 4)}}
     return obj, nn * a["steps"] / (fwd_ms * 1e-3) / 1e6
 
