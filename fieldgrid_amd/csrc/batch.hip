@@ -193,14 +193,16 @@ __device__ __forceinline__ bool persist_take(const BatchBlock *__restrict__ item
     pos = base + idx - p * qlen;
     run = items[pos].run;
     const uint32_t need = p * per_pass[run];
+    // bounded twice: by the wall clock, and by a poll count that does not depend on the clock's
+    // rate being what the runtime reports (2^24 polls of at least one s_sleep each: over a second)
     const uint64_t t0 = uint64_t(wall_clock64());
-    while (__hip_atomic_load(&done[run], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+    for (uint32_t polls = 0; __hip_atomic_load(&done[run], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; polls++) {
         if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-        if (uint64_t(wall_clock64()) - t0 > limit) {
+        if (uint64_t(wall_clock64()) - t0 > limit || polls >= (1u << 24)) {
             __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(8);
     }
     return true;
 }
@@ -362,6 +364,12 @@ __global__ __launch_bounds__(256) void k_spectra_flat(const float *__restrict__ 
 // ---- host -----------------------------------------------------------------------------------
 
 constexpr int N_LISTS = 24;  // thin wires (2) x CPML class (3) x wide index (2) x LDS sets (2)
+
+// The persistent launches (k_march2_persist) are switched off: their first GPU run (a 5-run
+// batch of small grids, tuned with persistent launches) did not finish, and its cause is not
+// known. The per-pass launches serve every batch meanwhile, and the sessions build no observer
+// captures (which only the persistent launches read).
+constexpr bool PERSIST_ENABLED = false;
 
 struct Batch {
     fg_device *dev = nullptr;
@@ -760,10 +768,6 @@ static double time_steps(Batch &b, uint64_t &at, uint32_t warm, uint32_t steps, 
 // once (a cooperative launch), one per CU slot that all four instances fit, a multiple of the
 // XCD count so each queue has as many workgroups.
 static void persist_setup(Batch &b, uint32_t count) {
-    // Switched off: the first GPU run of these launches (a 5-run batch of small grids, tuned
-    // with persistent launches) did not finish, and its cause is not known. The per-pass
-    // launches serve every batch meanwhile.
-    constexpr bool PERSIST_ENABLED = false;
     if (!PERSIST_ENABLED) return;
     int coop = 0, khz = 0;
     FG_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b.dev->ordinal));
@@ -1046,7 +1050,7 @@ static void make_sessions(Batch &b, uint32_t count, const fg_problem *const *pro
         try {
             FG_CHECK(hipSetDevice(b.dev->ordinal));
             for (uint32_t r; (r = next.fetch_add(1)) < count;)
-                b.runs[r].reset(session_new(b.dev, problems[r], driven[r], &pulses[r], &xs[r], true));
+                b.runs[r].reset(session_new(b.dev, problems[r], driven[r], &pulses[r], &xs[r], PERSIST_ENABLED));
         } catch (...) {
             std::lock_guard<std::mutex> lock(m);
             if (!first) first = std::current_exception();
@@ -1118,7 +1122,7 @@ extern "C" int fg_run_batch(fg_device *dev, uint32_t count, const fg_problem *co
             if (t.fix.n_debye) t.fix2.dsrc = s.deb2_jn.p, t.fix2.dold = s.deb2_eo.p;
             // the per-pass launches hand observers no captures: no plane DFT or NTFF in a pair
             if (!s.fuse2 || s.obs.plane_n || s.obs.n_patches) all_pairs = false;
-            if (s.fuse2)
+            if (s.fuse2 && s.cap_obs.p)
                 t.cp = Cap2{s.cap_start.p, s.cap_seg.p, s.cap_buf.p, s.cap_ntff.p, s.cap_plane.p, s.cap_obs.p,
                             s.cap_loop0, s.cap_probe0};
             b.seg2_cap = std::min(b.seg2_cap, march2_seg_cap(march2_plane_bytes(s.n, s.pml)));
