@@ -134,7 +134,7 @@ class fg_spectra(C.Structure):
 
 class fg_design(C.Structure):
     _fields_ = [("lo", u32 * 3), ("hi", u32 * 3), ("n_obs", u32), ("every", u32), ("obs", P(fg_point_probe)),
-                ("steps", u64)]
+                ("steps", u64), ("whole_tape", u32), ("pad", u32)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, u64, f64)

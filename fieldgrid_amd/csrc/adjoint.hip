@@ -1224,6 +1224,8 @@ struct fg_adjoint {
     fg::DevBuf<uint64_t> d_edges, d_obs;
     fg::DevBuf<int32_t> dmap;
     fg::DevBuf<float2> tape[2];  // design-edge tapes of two segments (recompute k-1 while the adjoint reads k)
+    bool whole = false;          // fg_design.whole_tape: the forward tapes every step, no recompute
+    fg::DevBuf<float2> tape_all;  // with `whole`: the tape of every step, step n at n * design edges
     fg::DevBuf<double> gca, gcb, dseries;
     uint64_t n_snaps = 0, snap_len = 0;
     double ms[4] = {0, 0, 0, 0};
@@ -1299,12 +1301,97 @@ static void set_box(fg_adjoint &A, const float *eps_box) {
 
 static Geo geo(const fg_adjoint &A) { return A.s->geo; }
 
+// The design edges' flat indices, the observations' flat indices and the map from (component,
+// box node) to design edge, on the device (once).
+static void design_tables(fg_adjoint &A) {
+    if (A.d_edges.p) return;
+    Session &s = *A.s;
+    hipStream_t st = s.stream;
+    const uint64_t nn = s.nn, ne = A.edges.size(), n_obs = A.obs.size();
+    std::vector<uint64_t> f(ne), o(n_obs);
+    for (uint64_t e = 0; e < ne; e++) f[e] = A.edges[e].f;
+    for (uint64_t q = 0; q < n_obs; q++)
+        o[q] = A.obs[q].field * nn + (uint64_t(A.obs[q].i) * s.n[1] + A.obs[q].j) * s.n[2] + A.obs[q].k;
+    std::vector<uint64_t> fe(f);
+    if (fe.empty()) fe.push_back(0);  // a box with no design edge: the tables exist, ne stays 0
+    A.d_edges.upload(fe.data(), fe.size(), st);
+    A.d_obs.upload(o.data(), o.size(), st);
+    // design edge per (component, node of the box's nodes lo..hi)
+    const fg_design &d = A.design;
+    uint64_t nb = 1;
+    for (int a = 0; a < 3; a++) nb *= d.hi[a] - d.lo[a] + 1;
+    std::vector<int32_t> map(3 * nb, -1);
+    for (uint64_t e = 0; e < ne; e++) {
+        const uint64_t c = f[e] / nn, id = f[e] % nn;
+        const uint64_t i = id / (uint64_t(s.n[1]) * s.n[2]), j = (id / s.n[2]) % s.n[1], k = id % s.n[2];
+        const uint64_t b = ((i - d.lo[0]) * (d.hi[1] - d.lo[1] + 1) + (j - d.lo[1])) * (d.hi[2] - d.lo[2] + 1) + (k - d.lo[2]);
+        map[c * nb + b] = int32_t(e);
+    }
+    A.dmap.upload(map.data(), map.size(), st);
+    FG_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+}
+
+// The forward steps from the state in copy 0 at step `start` to step `end`, two per launch
+// where the session allows it, taping (E^n, curl^n) on the design edges of each step into
+// tape + (n - start) * design edges: k_adj_tape2 for a two-step launch, k_adj_tape for one.
+static void steps_with_tape(fg_adjoint &A, uint64_t start, uint64_t end, float2 *tape) {
+    Session &s = *A.s;
+    hipStream_t st = s.stream;
+    const uint64_t ne = A.edges.size();
+    const uint32_t eb = uint32_t((ne + 255) / 256);
+    const Geo g = A.s->geo;
+    s.cur = 0;
+    s.steps = start;
+    set_step_counter(s);
+    for (uint64_t n = start; n < end;) {
+        const int prev = s.cur;
+        if (s.fuse2 && n + 2 <= end) {
+            launch_pair(s, uint32_t(n - start));
+            if (ne)
+                hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
+                                   s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
+                                   tape + (n - start) * ne, tape + (n + 1 - start) * ne);
+            n += 2;
+        } else {
+            launch_step(s, uint32_t(n - start), n);
+            if (ne)
+                hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
+                                   s.H[s.cur].p, tape + (n - start) * ne);
+            n += 1;
+        }
+    }
+    s.steps = end;
+}
+
 static void forward(fg_adjoint &A, float *series) {
     if (!A.built) rebuild(A);
     Session &s = *A.s;
     hipStream_t st = s.stream;
     reset(s);
     s.observe = true;
+    if (A.whole) {
+        // the tape of every step, no snapshots: the backward recomputes nothing
+        design_tables(A);
+        const uint64_t ne = A.edges.size(), len = std::max<uint64_t>(A.steps * ne, 1);
+        if (A.tape_all.cap < len) {
+            size_t free_b = 0, total_b = 0;
+            FG_CHECK(hipMemGetInfo(&free_b, &total_b));
+            if (len * 8 > free_b)
+                throw HipError{"the whole tape (" + std::to_string((len * 8) >> 20) + " MiB) does not fit in this GPU",
+                               FG_TOO_LARGE};
+            A.tape_all.alloc(len);
+        }
+        A.n_snaps = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        steps_with_tape(A, 0, A.steps, A.tape_all.p);
+        FG_CHECK(hipGetLastError());
+        FG_CHECK(hipStreamSynchronize(st));
+        A.ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (series && A.steps && !A.obs.empty())
+            FG_CHECK(hipMemcpy(series, s.probes.p, A.steps * A.obs.size() * 4, hipMemcpyDeviceToHost));
+        A.have_forward = true;
+        return;
+    }
     const uint64_t psi_n = s.psi[0].n, nn = s.nn;
     A.snap_len = 6 * nn + psi_n + A.dlen + A.ilen;
     A.n_snaps = (A.steps + A.every - 1) / A.every;  // segment k starts at k*every; k = 0 is all zeros
@@ -1430,6 +1517,7 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     hipStream_t st = s.stream;
     const uint64_t nn = s.nn, psi_n = s.psi[0].n, ne = A.edges.size(), n_obs = A.obs.size();
     const Geo g = geo(A);
+    design_tables(A);
     if (A.lamE.n != 3 * nn) {
         A.lamE.alloc(3 * nn);
         A.lamA.alloc(3 * nn);
@@ -1437,28 +1525,12 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
         A.lamH2.alloc(3 * nn);
         A.lamP[0].alloc(std::max<uint64_t>(psi_n, 1));
         A.lamP[1].alloc(std::max<uint64_t>(psi_n, 1));
-        std::vector<uint64_t> f(ne), o(n_obs);
-        for (uint64_t e = 0; e < ne; e++) f[e] = A.edges[e].f;
-        for (uint64_t q = 0; q < n_obs; q++)
-            o[q] = A.obs[q].field * nn + (uint64_t(A.obs[q].i) * s.n[1] + A.obs[q].j) * s.n[2] + A.obs[q].k;
-        A.d_edges.upload(f.data(), f.size(), st);
-        A.d_obs.upload(o.data(), o.size(), st);
-        A.tape[0].alloc(std::max<uint64_t>(A.every * ne, 1));
-        A.tape[1].alloc(std::max<uint64_t>(A.every * ne, 1));
         A.gca.alloc(std::max<uint64_t>(ne, 1));
         A.gcb.alloc(std::max<uint64_t>(ne, 1));
-        // design edge per (component, node of the box's nodes lo..hi)
-        const fg_design &d = A.design;
-        uint64_t nb = 1;
-        for (int a = 0; a < 3; a++) nb *= d.hi[a] - d.lo[a] + 1;
-        std::vector<int32_t> map(3 * nb, -1);
-        for (uint64_t e = 0; e < ne; e++) {
-            const uint64_t c = f[e] / nn, id = f[e] % nn;
-            const uint64_t i = id / (uint64_t(s.n[1]) * s.n[2]), j = (id / s.n[2]) % s.n[1], k = id % s.n[2];
-            const uint64_t b = ((i - d.lo[0]) * (d.hi[1] - d.lo[1] + 1) + (j - d.lo[1])) * (d.hi[2] - d.lo[2] + 1) + (k - d.lo[2]);
-            map[c * nb + b] = int32_t(e);
-        }
-        A.dmap.upload(map.data(), map.size(), st);
+    }
+    if (!A.whole && A.tape[0].cap < std::max<uint64_t>(A.every * ne, 1)) {
+        A.tape[0].alloc(std::max<uint64_t>(A.every * ne, 1));
+        A.tape[1].alloc(std::max<uint64_t>(A.every * ne, 1));
     }
     const bool media = A.dlen || A.ilen || s.wires.n;
     if (media) {
@@ -1492,12 +1564,13 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     const char *ov = getenv("FIELDGRID_ADJ_OVERLAP");
     if (!A.astream) FG_CHECK(hipStreamCreateWithFlags(&A.astream, hipStreamNonBlocking));
     hipStream_t as = ov && ov[0] == '0' ? st : A.astream;
-    // per segment: recompute start/end on st, adjoint start/end on as; plus the setup event
-    std::vector<hipEvent_t> ev(4 * A.n_snaps + 1);
+    // per segment: recompute start/end on st, adjoint start/end on as; plus the setup event.
+    // With the whole tape there is one segment, all steps, and nothing to recompute.
+    const uint64_t segs = A.whole ? 1 : A.n_snaps;
+    std::vector<hipEvent_t> ev(4 * segs + 1);
     for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
-    hipEvent_t ready = ev[4 * A.n_snaps];
+    hipEvent_t ready = ev[4 * segs];
     const dim3 block(64, 4), grid((g.n2 + 63) / 64, (g.n1 + 3) / 4, g.n0);
-    const uint32_t eb = uint32_t((ne + 255) / 256);
     GradArgs ga{};
     InjectArgs in{A.d_obs.p, nullptr, uint32_t(n_obs)};
     for (int a = 0; a < 3; a++) ga.lo[a] = A.design.lo[a], ga.nb[a] = A.design.hi[a] - A.design.lo[a] + 1;
@@ -1510,48 +1583,32 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     if (n_obs && A.steps)
         hipLaunchKernelGGL(k_adj_inject, dim3(1), dim3(64), 0, as, g, A.lamE.p, march ? nullptr : A.lamA.p, A.d_obs.p,
                            uint32_t(n_obs), A.dseries.p + (A.steps - 1) * n_obs);
-    for (uint64_t k = A.n_snaps; k-- > 0;) {
-        const uint64_t start = k * A.every, end = std::min(start + A.every, A.steps);
-        float2 *tape = A.tape[k & 1].p;
-        if (k + 2 < A.n_snaps) FG_CHECK(hipStreamWaitEvent(st, ev[4 * (k + 2) + 3], 0));
+    for (uint64_t k = segs; k-- > 0;) {
+        const uint64_t start = A.whole ? 0 : k * A.every, end = A.whole ? A.steps : std::min(start + A.every, A.steps);
+        float2 *tape = A.whole ? A.tape_all.p : A.tape[k & 1].p;
+        if (k + 2 < segs) FG_CHECK(hipStreamWaitEvent(st, ev[4 * (k + 2) + 3], 0));
         FG_CHECK(hipEventRecord(ev[4 * k], st));
-        // restore the segment's starting fields into copy 0
-        if (k == 0) {
-            s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
-            if (A.dlen) s.dstate.zero(st);
-            if (A.ilen) FG_CHECK(hipMemcpyAsync(s.ind_i.p, s.ind_i0.data(), A.ilen * 4, hipMemcpyHostToDevice, st));
-        } else {
-            const float *src = A.snaps.p + (k - 1) * A.snap_len;
-            FG_CHECK(hipMemcpyAsync(s.E[0].p, src, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
-            FG_CHECK(hipMemcpyAsync(s.H[0].p, src + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
-            if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, src + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
-            if (A.dlen) FG_CHECK(hipMemcpyAsync(s.dstate.p, src + 6 * nn + psi_n, A.dlen * 4, hipMemcpyDeviceToDevice, st));
-            if (A.ilen)
-                FG_CHECK(hipMemcpyAsync(s.ind_i.p, src + 6 * nn + psi_n + A.dlen, A.ilen * 4, hipMemcpyDeviceToDevice, st));
-        }
-        s.cur = 0;
-        s.steps = start;
-        s.observe = false;  // the series of this segment is already known
-        set_step_counter(s);
-        for (uint64_t n = start; n < end;) {
-            const int prev = s.cur;
-            if (s.fuse2 && n + 2 <= end) {  // two steps per launch, both taped
-                launch_pair(s, uint32_t(n - start));
-                if (ne)
-                    hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
-                                       s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
-                                       tape + (n - start) * ne, tape + (n + 1 - start) * ne);
-                n += 2;
+        if (!A.whole) {
+            // restore the segment's starting fields into copy 0
+            if (k == 0) {
+                s.E[0].zero(st), s.H[0].zero(st), s.psi[0].zero(st);
+                if (A.dlen) s.dstate.zero(st);
+                if (A.ilen) FG_CHECK(hipMemcpyAsync(s.ind_i.p, s.ind_i0.data(), A.ilen * 4, hipMemcpyHostToDevice, st));
             } else {
-                launch_step(s, uint32_t(n - start), n);
-                if (ne)
-                    hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
-                                       s.H[s.cur].p, tape + (n - start) * ne);
-                n += 1;
+                const float *src = A.snaps.p + (k - 1) * A.snap_len;
+                FG_CHECK(hipMemcpyAsync(s.E[0].p, src, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+                FG_CHECK(hipMemcpyAsync(s.H[0].p, src + 3 * nn, 3 * nn * 4, hipMemcpyDeviceToDevice, st));
+                if (psi_n) FG_CHECK(hipMemcpyAsync(s.psi[0].p, src + 6 * nn, psi_n * 4, hipMemcpyDeviceToDevice, st));
+                if (A.dlen)
+                    FG_CHECK(hipMemcpyAsync(s.dstate.p, src + 6 * nn + psi_n, A.dlen * 4, hipMemcpyDeviceToDevice, st));
+                if (A.ilen)
+                    FG_CHECK(hipMemcpyAsync(s.ind_i.p, src + 6 * nn + psi_n + A.dlen, A.ilen * 4, hipMemcpyDeviceToDevice,
+                                            st));
             }
+            s.observe = false;  // the series of this segment is already known
+            steps_with_tape(A, start, end, tape);
+            s.observe = true;
         }
-        s.steps = end;
-        s.observe = true;
         FG_CHECK(hipEventRecord(ev[4 * k + 1], st));
         FG_CHECK(hipStreamWaitEvent(as, ev[4 * k + 1], 0));
         FG_CHECK(hipEventRecord(ev[4 * k + 2], as));
@@ -1592,12 +1649,13 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     FG_CHECK(hipStreamSynchronize(st));
     A.ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     A.ms[2] = A.ms[3] = 0.0;  // busy time of each stream (they overlap)
-    for (uint64_t k = 0; k < A.n_snaps; k++) {
+    for (uint64_t k = 0; k < segs; k++) {
         float a = 0.f, b = 0.f;
         FG_CHECK(hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]));
         FG_CHECK(hipEventElapsedTime(&b, ev[4 * k + 2], ev[4 * k + 3]));
         A.ms[2] += a, A.ms[3] += b;
     }
+    if (A.whole) A.ms[2] = 0.0;  // nothing recomputed (the events only bracket an empty interval)
     for (auto &e : ev) (void)hipEventDestroy(e);
     // chain rule per design edge: ca = (1-x)/(1+x), cb = dt/(eps (1+x)), x = sig dt / (2 eps)
     // (sim.rs:431-441), eps = EPS0 (fixed + sum w eps_r)
@@ -1729,6 +1787,7 @@ int fg_adjoint_new(fg_device *dev, const fg_sim_desc *desc, const fg_design *des
             if (design->lo[a] >= design->hi[a] || design->hi[a] > desc->len[a] - 1)
                 throw HipError{"the design box lies outside the grid", FG_INVALID};
         A->design.obs = A->obs.data();
+        A->whole = design->whole_tape != 0;
         A->box_cells = uint64_t(design->hi[0] - design->lo[0]) * (design->hi[1] - design->lo[1]) * (design->hi[2] - design->lo[2]);
         A->steps = design->steps;
         A->every = design->every ? design->every : 100;

@@ -654,10 +654,12 @@ class Design:
     extension without counterpart in the reference). The contract of a torch.autograd.Function:
     forward(eps_box) -> series of the observed E components after every step, shape
     (steps, n_obs); backward(dseries) -> dJ/d eps_r for the box, given dJ/d series.
-    `obs` = [(field 0..2, i, j, k)]; the box [lo, hi) is in cells."""
+    `obs` = [(field 0..2, i, j, k)]; the box [lo, hi) is in cells. `whole_tape`: the forward
+    tapes the design edges of every step and the backward recomputes nothing (else
+    checkpoint-and-recompute from snapshots every `every` steps)."""
 
     def __init__(self, gpu: Gpu, grid, mats, pec, lumped, ports, resistive, media, lo, hi, obs, driven,
-                 pulse: Pulse, steps: int, every: int = 100):
+                 pulse: Pulse, steps: int, every: int = 100, whole_tape: bool = False):
         if gpu is None:
             raise ValueError("Design needs a Gpu")
         self.gpu = gpu  # the device outlives the adjoint's sessions
@@ -669,6 +671,7 @@ class Design:
         d = abi.fg_design()
         d.lo[:], d.hi[:] = list(lo), list(hi)
         d.n_obs, d.every, d.obs, d.steps = len(obs), every, self._obs, steps
+        d.whole_tape = 1 if whole_tape else 0
         self._design = d
         h = C.c_void_p()
         abi.check(abi.lib().fg_adjoint_new(gpu._h, C.byref(self.desc.desc), C.byref(d), driven,

@@ -17,7 +17,9 @@
  * snapshot of the fields every `every` steps. The backward pass recomputes one segment at a time
  * from its snapshot, taping the design-edge values, and runs the adjoint over that segment on a
  * second stream while the next (earlier) segment is recomputed. Memory is the snapshots plus two
- * segments' tapes.
+ * segments' tapes. With fg_design.whole_tape the forward tapes every step instead and the
+ * backward runs the adjoint alone: more memory where the design box is large against the grid,
+ * no recompute.
  *
  * Supported problems: every Sim feature but resistive sheets, whose face weights follow the H^2
  * the run has accumulated (cpu.rs:169-193), so a step with sheets is not linear in the fields:
@@ -45,6 +47,12 @@ typedef struct fg_design {
     uint32_t every;             /* steps between field snapshots (0 = 100)                     */
     const fg_point_probe *obs;  /* observed E components (field 0..2) at nodes               */
     uint64_t steps;             /* steps of every forward run                                   */
+    uint32_t whole_tape;        /* 0: checkpoint-and-recompute (snapshots every `every` steps);
+                                   1: the forward tapes the design edges of every step (8 B per
+                                   design edge per step, in place of the snapshots) and the
+                                   backward recomputes nothing; FG_TOO_LARGE if that tape does
+                                   not fit in free device memory                               */
+    uint32_t pad;
 } fg_design;
 
 typedef struct fg_adjoint fg_adjoint;
@@ -53,8 +61,9 @@ typedef struct fg_adjoint fg_adjoint;
    the arrays it points at must stay valid until fg_adjoint_free; the eps array is copied. */
 int fg_adjoint_new(fg_device *dev, const fg_sim_desc *desc, const fg_design *design, uint32_t driven,
                    const fg_pulse *pulse, fg_adjoint **out);
-/* Forward run with snapshots. eps_box (k fastest, NULL = the values in desc or the last call)
-   replaces the box's eps_r; series: steps * n_obs floats. */
+/* Forward run with snapshots (or, with whole_tape, the tape of every step). eps_box (k fastest,
+   NULL = the values in desc or the last call) replaces the box's eps_r; series: steps * n_obs
+   floats. */
 int fg_adjoint_forward(fg_adjoint *adj, const float *eps_box, float *series);
 /* Backward pass for the last forward run: dseries (steps * n_obs doubles, dJ/ds) -> grad
    (box cells, k fastest, dJ/d eps_r). */
@@ -64,7 +73,8 @@ int fg_adjoint_forward64(fg_adjoint *adj, const double *eps_box, double *series)
 /* Wall times (ms) of the last forward and backward passes, then the busy times (HIP events) of
    the backward's two streams: the recompute of the segments and the adjoint steps. The backward
    recomputes segment k-1 while the adjoint sweeps segment k, so the two overlap and their sum
-   exceeds the backward's wall time. Also the number of snapshots kept. */
+   exceeds the backward's wall time (with whole_tape there is no recompute: 0). Also the number
+   of snapshots kept (0 with whole_tape). */
 int fg_adjoint_timing(const fg_adjoint *adj, double ms[4], uint64_t *snapshots);
 void fg_adjoint_free(fg_adjoint *adj);
 

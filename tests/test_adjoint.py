@@ -22,10 +22,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def make(gpu, args, every=100):
+def make(gpu, args, every=100, whole_tape=False):
     return F.Design(gpu, args["grid"], args["mats"], args["pec"], args["lumped"], args["ports"], args["resistive"],
                     args["media"], args["lo"], args["hi"], args["obs"], args["driven"], args["pulse"], args["steps"],
-                    every)
+                    every, whole_tape)
 
 
 def box_of(args):
@@ -297,3 +297,32 @@ def test_the_media_state_survives_the_snapshots(gpu):
         d.close()
     assert np.max(np.abs(grads[0])) > 0
     assert np.array_equal(grads[0], grads[1])
+
+
+# ---- the whole tape: the forward tapes every step, the backward recomputes nothing ---------------
+
+def test_the_whole_tape_gives_the_checkpointed_gradient(gpu):
+    """fg_design.whole_tape against checkpoint-and-recompute, bit for bit (series and gradient):
+    the tape holds the same design-edge values whichever pass writes it, and the adjoint adds
+    them in the same order. On the small design problem with and without media (a Debye block,
+    an inductor, a thin wire: one step per pass in the adjoint), with odd step counts, and on a
+    grid with every block class."""
+    small, freq_s = W.design_small(steps=157)
+    media, _ = W.design_small(steps=131, media=True)
+    big, _ = W.design(dims=(40, 64, 140), dx=1e-3, pml=5, box=((12, 20), (20, 30), (60, 80)), port=(9, 30),
+                      probe=(30, 40), steps=130, sigma=0.02)
+    for name, args in (("small", small), ("media", media), ("40x64x140", big)):
+        ck, wt = make(gpu, args, every=40), make(gpu, args, every=40, whole_tape=True)
+        try:
+            s1, s2 = ck.forward(), wt.forward()
+            assert np.array_equal(s1, s2), name
+            _, dJ = W.dft_objective(s1[:, 0], F.time_step(args["grid"]), freq_of(args))
+            g1, g2 = ck.backward(dJ), wt.backward(dJ)
+            (_, _, rec, _), snaps = wt.timing()
+            print(f"\n{name}: max |grad| {np.max(np.abs(g1)):.3e}, whole-tape snapshots {snaps}, recompute {rec:.1f} ms")
+            assert np.max(np.abs(g1)) > 0
+            assert np.array_equal(g1, g2), name
+            assert snaps == 0 and rec == 0.0
+        finally:
+            ck.close()
+            wt.close()
