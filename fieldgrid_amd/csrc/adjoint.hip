@@ -1230,6 +1230,10 @@ struct fg_adjoint {
     uint64_t n_snaps = 0, snap_len = 0;
     double ms[4] = {0, 0, 0, 0};
     hipStream_t astream = nullptr;  // the adjoint sweeps' stream (the session stream recomputes)
+    // the x/y-slab class of a two-step adjoint pass runs on xy_stream, forked from and joined
+    // back to the adjoint stream, next to the other classes (each is about one round of blocks)
+    hipStream_t xy_stream = nullptr;
+    hipEvent_t xy_fork = nullptr, xy_join = nullptr;
     // block lists of the two-step adjoint pass (k_adj_march2), per CPML class, for segments of
     // adj_seg planes
     fg::DevBuf<fg::MarchBlock> ablk2[3];
@@ -1241,6 +1245,9 @@ struct fg_adjoint {
 
     ~fg_adjoint() {
         if (astream) (void)hipStreamSynchronize(astream), (void)hipStreamDestroy(astream);
+        if (xy_stream) (void)hipStreamSynchronize(xy_stream), (void)hipStreamDestroy(xy_stream);
+        for (hipEvent_t e : {xy_fork, xy_join})
+            if (e) (void)hipEventDestroy(e);
         s.reset();
         if (sim) fg_sim_free(sim);
     }
@@ -1457,7 +1464,8 @@ static void adj_march(Session &s, hipStream_t st, const float *Lo, float *Ln, co
 constexpr uint32_t ADJ_TJ_XY = 6;
 
 // Two adjoint steps through k_adj_march2 over the two-step adjoint block lists (march2's tile,
-// reading one plane further along i), one launch per CPML class on the stream.
+// reading one plane further along i), one launch per CPML class: the x/y-slab class on
+// xy_stream next to the others (FIELDGRID_ADJ_STREAMS=0: all on the stream, one after another).
 static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln, const float *Mo, float *Mn,
                        const float *po, float *pn, const GradArgs &ga, const float2 *tapeB, const InjectArgs &in,
                        const double *dsB) {
@@ -1486,19 +1494,32 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
         }
         FG_CHECK(hipStreamSynchronize(s.stream));
         A.adj_seg = sg;
+        const char *cs = getenv("FIELDGRID_ADJ_STREAMS");
+        if (!A.xy_stream && !(cs && cs[0] == '0') && !cls[2].empty() && !(cls[0].empty() && cls[1].empty())) {
+            FG_CHECK(hipStreamCreateWithFlags(&A.xy_stream, hipStreamNonBlocking));
+            FG_CHECK(hipEventCreateWithFlags(&A.xy_fork, hipEventDisableTiming));
+            FG_CHECK(hipEventCreateWithFlags(&A.xy_join, hipEventDisableTiming));
+        }
     }
     const dim3 block(64, 12 + 4), block_xy(64, ADJ_TJ_XY + 4);
     const uint32_t n0 = uint32_t(A.ablk2[0].n), n1 = uint32_t(A.ablk2[1].n), n2 = uint32_t(A.ablk2[2].n);
     const MarchBlock *b0 = A.ablk2[0].p, *b1 = A.ablk2[1].p, *b2 = A.ablk2[2].p;
     const uint32_t sg = A.adj_seg;
+    // every node is owned by one block of one class, so the classes may run in any order
+    hipStream_t xs = st;
+    if (A.xy_stream && n2) {
+        FG_CHECK(hipEventRecord(A.xy_fork, st));
+        FG_CHECK(hipStreamWaitEvent(A.xy_stream, A.xy_fork, 0));
+        xs = A.xy_stream;
+    }
 #define FG_AM2(W, L)                                                                                             \
     do {                                                                                                         \
-        if (n0) hipLaunchKernelGGL((k_adj_march2<12, 0, W, L>), dim3(n0), block, 0, st, g, b0, Lo, Ln, Mo, Mn, po, pn, \
-                                   sg, ga, tapeB, in, dsB);                                                      \
+        if (n2) hipLaunchKernelGGL((k_adj_march2<ADJ_TJ_XY, 2, W, L>), dim3(n2), block_xy, 0, xs, g, b2, Lo, Ln, Mo, \
+                                   Mn, po, pn, sg, ga, tapeB, in, dsB);                                          \
         if (n1) hipLaunchKernelGGL((k_adj_march2<12, 1, W, L>), dim3(n1), block, 0, st, g, b1, Lo, Ln, Mo, Mn, po, pn, \
                                    sg, ga, tapeB, in, dsB);                                                      \
-        if (n2) hipLaunchKernelGGL((k_adj_march2<ADJ_TJ_XY, 2, W, L>), dim3(n2), block_xy, 0, st, g, b2, Lo, Ln, Mo, \
-                                   Mn, po, pn, sg, ga, tapeB, in, dsB);                                          \
+        if (n0) hipLaunchKernelGGL((k_adj_march2<12, 0, W, L>), dim3(n0), block, 0, st, g, b0, Lo, Ln, Mo, Mn, po, pn, \
+                                   sg, ga, tapeB, in, dsB);                                                      \
     } while (0)
     const bool lds = g.n_sets <= uint32_t(SETS_LDS);
     if (g.wide) {
@@ -1509,6 +1530,10 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
         else FG_AM2(false, false);
     }
 #undef FG_AM2
+    if (xs != st) {
+        FG_CHECK(hipEventRecord(A.xy_join, xs));
+        FG_CHECK(hipStreamWaitEvent(st, A.xy_join, 0));
+    }
 }
 
 static void backward(fg_adjoint &A, const double *dseries, double *grad) {
