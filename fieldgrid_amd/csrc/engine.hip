@@ -531,6 +531,11 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
             }
 }
 
+// the segment tuner of the two-step march: runs of at least SEG2_TUNE_MIN_STEPS steps time
+// SEG2_TUNE_PAIRS passes of each candidate
+constexpr uint64_t SEG2_TUNE_MIN_STEPS = 800;
+constexpr uint32_t SEG2_TUNE_PAIRS = 3;
+
 static void plan_march2(Session &s) {
     std::vector<MarchBlock> cls[3];
     march_classes2(s, s.tj2, s.seg2, cls);
@@ -1000,6 +1005,17 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
                 if (2 * (cls[0].size() + cls[1].size() + cls[2].size()) >= 3 * cus) break;
                 s.seg2 /= 2;
             }
+            // The best length also depends on how evenly the segments cut the i extent and on
+            // how the side work spreads over the blocks (measured at 256^3 with Debye media and
+            // NTFF: 16 planes 4 % faster than 32; at 768x768x384: 24 planes 0.5 % faster): a
+            // long run times a few passes of seg2, 3/4, 1/2, 3/8 and 1/4 of it first.
+            const char *tu = getenv("FIELDGRID_SEG2_TUNE");
+            if (!(tu && tu[0] == '0') && X->max_steps >= SEG2_TUNE_MIN_STEPS)
+                for (uint32_t num : {8u, 6u, 4u, 3u, 2u}) {
+                    const uint32_t c = s.seg2 * num / 8;
+                    if (c >= 4 && (s.seg2_try.empty() || s.seg2_try.back() != c)) s.seg2_try.push_back(c);
+                }
+            if (s.seg2_try.size() < 2) s.seg2_try.clear();
         }
         // the driven port's sources by plane (i), for the first step inside the launch
         std::vector<std::vector<uint4>> by_plane(P->n[0]);
@@ -1307,7 +1323,48 @@ void set_step_counter(Session &s) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
 }
 
+// Time SEG2_TUNE_PAIRS two-step passes of each segment length in s.seg2_try (after one untimed
+// pass), keep the fastest and re-plan the march with it. The passes are steps of the run: every
+// tiling computes the same fields bit for bit. Returns the steps taken.
+static uint64_t tune_seg2(Session &s) {
+    std::vector<uint32_t> cand;
+    cand.swap(s.seg2_try);
+    const uint32_t pairs = SEG2_TUNE_PAIRS;
+    std::vector<hipEvent_t> ev(2 * cand.size());
+    for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
+    auto chunk = [&](uint64_t k) {
+        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
+        launch_steps(s, k);
+        s.steps += k;
+    };
+    chunk(2);
+    for (size_t c = 0; c < cand.size(); c++) {
+        s.seg2 = cand[c];
+        plan_march2(s);
+        FG_CHECK(hipEventRecord(ev[2 * c], s.stream));
+        chunk(2 * pairs);
+        FG_CHECK(hipEventRecord(ev[2 * c + 1], s.stream));
+    }
+    FG_CHECK(hipStreamSynchronize(s.stream));
+    size_t best = 0;
+    float best_ms = 0.f;
+    for (size_t c = 0; c < cand.size(); c++) {
+        float ms = 0.f;
+        FG_CHECK(hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]));
+        if (c == 0 || ms < best_ms) best = c, best_ms = ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (best + 1 != cand.size()) {
+        s.seg2 = cand[best];
+        plan_march2(s);
+        FG_CHECK(hipStreamSynchronize(s.stream));
+    }
+    return 2 + 2ull * pairs * cand.size();
+}
+
 void enqueue_steps(Session &s, uint64_t count) {
+    if (!s.timing && !s.seg2_try.empty() && count >= 2 + 2ull * SEG2_TUNE_PAIRS * s.seg2_try.size())
+        count -= tune_seg2(s);
     if (s.timing) {
         collect_timing(s);
         // timed steps go in batches so the event pool stays bounded

@@ -223,6 +223,9 @@ struct Session {
     // two steps per launch (march2): on when nothing but sources sits between two steps
     bool fuse2 = false;
     uint32_t tj2 = 12, seg2 = 32;
+    // segment lengths for the two-step march to time at the start of the run (enqueue_steps
+    // picks the fastest and drops the list); empty: seg2 as set
+    std::vector<uint32_t> seg2_try;
     DevBuf<MarchBlock> blk2_int, blk2_z, blk2_pml;
     DevBuf<MarchBlock> blk2_mix;  // the three, class-tagged, for k_march2_mix
     bool mix2 = true;             // one mixed launch per pass where the classes would run serially

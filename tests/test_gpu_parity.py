@@ -333,6 +333,31 @@ def test_two_steps_per_launch_change_nothing(gpu, monkeypatch, seg2, dispatch):
     _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"layered seg2 {seg2} {dispatch}")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("check_every", [1000, 37])
+def test_the_segment_tuner_changes_nothing(gpu, monkeypatch, check_every):
+    """A run of 800 steps or more times a few two-step passes of several segment lengths before
+    it goes on with the fastest (the timed passes are steps of the run). The boxes recipe at
+    128^3 (8-plane segments by default, so 8, 6 and 4 are timed) with Debye boxes, NTFF and a
+    plane DFT over 840 steps, tuned and untuned, the tuner's 20 steps taken from a first chunk of
+    1000 or of 37 steps (an odd rest, so the chunk ends on a single step): every output
+    bit-identical, and equal to the FP32 oracle."""
+    sim = W.boxes(nodes=128, pml=8, nboxes=8, ndebye=4, per_box_tau=True)
+    pulse = W.boxes_pulse()
+    ex = W.boxes_extras(sim, steps=840, inset=2, check_every=check_every, plane_k=61,
+                        probes=[(2, 60, 58, 40), (3, 55, 70, 71)])
+    monkeypatch.delenv("FIELDGRID_SEG2", raising=False)
+    monkeypatch.delenv("FIELDGRID_SEG2_TUNE", raising=False)
+    tuned = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    monkeypatch.setenv("FIELDGRID_SEG2_TUNE", "0")
+    plain = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    for name in ("series", "probes", "plane", "ntff", "e", "h"):
+        a, b = getattr(tuned, name), getattr(plain, name)
+        assert a.size > 0 and np.array_equal(a, b), f"{name}: {int(np.sum(a != b))} of {a.size} differ"
+    if check_every == 1000:
+        _exact(tuned, O.run_cpu(sim.desc, 0, pulse, ex, 32), "boxes 128^3 with the segment tuner")
+
+
 # (n0, n1, n2) around the march tiles' edges: k extents just under, at and over one and two
 # 60-lane tiles and 64 lanes, row counts at and just over multiples of the 12- and 14-row tiles,
 # plane counts under one segment and over two
