@@ -994,8 +994,14 @@ __global__ __launch_bounds__(256) void k_adj_tape2(Geo g, FixArgs f, const uint6
         curl0 = (h_mid<1>(g, Eo, Ho, Po, i, j, k) - h_mid<1>(g, Eo, Ho, Po, i - 1, j, k)) * g.ax[0].inv_dd[i] -
                 (h_mid<0>(g, Eo, Ho, Po, i, j, k) - h_mid<0>(g, Eo, Ho, Po, i, j - 1, k)) * g.ax[1].inv_dd[j];
     }
-    tape0[e] = make_float2(Eo[fl], curl0);
-    const float e1 = field_mid(g, f, pl, Eo, Ho, Po, fl);
+    // step n+1's E^{n+1} = ca E^n + cb curl^n (e_mid's update with the curl just formed: a
+    // design edge is clear of the CPML, so no psi term enters), less its sources
+    const float eo = Eo[fl];
+    tape0[e] = make_float2(eo, curl0);
+    const float2 cf = coef_at(g, fl);
+    float e1 = cf.y != 0.f ? cf.x * eo + cf.y * curl0 : eo;
+    for (uint32_t q = 0; q < f.n_src; q++)
+        if (f.src[q].comp == c && f.src[q].id == id) e1 -= f.src[q].src * pl;
     const uint64_t sx = uint64_t(g.n1) * g.n2, sy = g.n2;
     float curl1;
     if (c == 0) {
@@ -1341,7 +1347,8 @@ static void design_tables(fg_adjoint &A) {
 // The forward steps from the state in copy 0 at step `start` to step `end`, two per launch
 // where the session allows it, taping (E^n, curl^n) on the design edges of each step into
 // tape + (n - start) * design edges: k_adj_tape2 for a two-step launch, k_adj_tape for one.
-static void steps_with_tape(fg_adjoint &A, uint64_t start, uint64_t end, float2 *tape) {
+// tune (the whole-tape forward): the segment tuner times its first taped passes.
+static void steps_with_tape(fg_adjoint &A, uint64_t start, uint64_t end, float2 *tape, bool tune = false) {
     Session &s = *A.s;
     hipStream_t st = s.stream;
     const uint64_t ne = A.edges.size();
@@ -1350,23 +1357,31 @@ static void steps_with_tape(fg_adjoint &A, uint64_t start, uint64_t end, float2 
     s.cur = 0;
     s.steps = start;
     set_step_counter(s);
-    for (uint64_t n = start; n < end;) {
-        const int prev = s.cur;
-        if (s.fuse2 && n + 2 <= end) {
-            launch_pair(s, uint32_t(n - start));
-            if (ne)
-                hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
-                                   s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
-                                   tape + (n - start) * ne, tape + (n + 1 - start) * ne);
-            n += 2;
-        } else {
-            launch_step(s, uint32_t(n - start), n);
-            if (ne)
-                hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
-                                   s.H[s.cur].p, tape + (n - start) * ne);
-            n += 1;
+    uint64_t n = start;
+    auto advance = [&](uint64_t upto) {
+        for (; n < upto;) {
+            const int prev = s.cur;
+            if (s.fuse2 && n + 2 <= end) {
+                launch_pair(s, uint32_t(n - start));
+                if (ne)
+                    hipLaunchKernelGGL(k_adj_tape2, dim3(eb), dim3(256), 0, st, g, s.fix, A.d_edges.p, uint32_t(ne),
+                                       s.E[prev].p, s.H[prev].p, s.psi[prev].p, s.H[s.cur].p, uint32_t(n - start),
+                                       tape + (n - start) * ne, tape + (n + 1 - start) * ne);
+                n += 2;
+            } else {
+                launch_step(s, uint32_t(n - start), n);
+                if (ne)
+                    hipLaunchKernelGGL(k_adj_tape, dim3(eb), dim3(256), 0, st, g, A.d_edges.p, uint32_t(ne), s.E[prev].p,
+                                       s.H[s.cur].p, tape + (n - start) * ne);
+                n += 1;
+            }
         }
-    }
+    };
+    // the whole-tape forward (one call over every step) times the two-step march's segment
+    // lengths on its first taped passes, as enqueue_steps does for a plain run
+    if (tune && s.fuse2 && !s.seg2_try.empty() && end - start >= seg2_tune_steps(s))
+        tune_seg2(s, [&](uint64_t k) { advance(n + k); });
+    advance(end);
     s.steps = end;
 }
 
@@ -1390,7 +1405,7 @@ static void forward(fg_adjoint &A, float *series) {
         }
         A.n_snaps = 0;
         const auto t0 = std::chrono::steady_clock::now();
-        steps_with_tape(A, 0, A.steps, A.tape_all.p);
+        steps_with_tape(A, 0, A.steps, A.tape_all.p, true);
         FG_CHECK(hipGetLastError());
         FG_CHECK(hipStreamSynchronize(st));
         A.ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1473,7 +1488,7 @@ static void adj_march2(fg_adjoint &A, hipStream_t st, const float *Lo, float *Ln
     const Geo &g = s.geo;
     if (!A.adj_seg) {
         const char *e = getenv("FIELDGRID_ADJ_SEG2");
-        const uint32_t sg = e ? uint32_t(std::max(1, std::min(atoi(e), SEG_MAX))) : s.seg2;
+        const uint32_t sg = e ? uint32_t(std::max(1, std::min(atoi(e), SEG_MAX))) : s.seg2_base;
         std::vector<MarchBlock> cls[3];
         march_classes2(s, 12, sg, cls, 1);
         // The x/y-slab class runs 6-row tiles (each 12-row tile as two): at 12 rows its 1024
@@ -1577,6 +1592,15 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     // wires take one step per pass (their transposes sit between the steps)
     const char *ap = getenv("FIELDGRID_ADJ_PAIRS");
     const bool pairs = march && !media && !(ap && ap[0] == '0');
+    // The recompute marches at the segment length set before any tuning: the forward's tuner
+    // times the forward alone, and the recompute shares the GPU with the adjoint sweep (on the
+    // 192x192x112 design grid the forward is fastest at shorter segments, the backward with a
+    // recompute at 16 planes). The tuned length is restored at the end.
+    const uint32_t tuned = s.seg2;
+    if (!A.whole && s.fuse2 && tuned != s.seg2_base) {
+        s.seg2 = s.seg2_base;
+        plan_march2(s);
+    }
     float *LE[2] = {A.lamE.p, A.lamA.p}, *LH[2] = {A.lamH.p, A.lamH2.p};
     int le = 0;
     A.gca.zero(st), A.gcb.zero(st);
@@ -1673,6 +1697,11 @@ static void backward(fg_adjoint &A, const double *dseries, double *grad) {
     FG_CHECK(hipStreamSynchronize(as));
     FG_CHECK(hipStreamSynchronize(st));
     A.ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (s.seg2 != tuned) {
+        s.seg2 = tuned;
+        plan_march2(s);
+        FG_CHECK(hipStreamSynchronize(st));
+    }
     A.ms[2] = A.ms[3] = 0.0;  // busy time of each stream (they overlap)
     for (uint64_t k = 0; k < segs; k++) {
         float a = 0.f, b = 0.f;

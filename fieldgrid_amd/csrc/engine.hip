@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -536,7 +537,7 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
 constexpr uint64_t SEG2_TUNE_MIN_STEPS = 800;
 constexpr uint32_t SEG2_TUNE_PAIRS = 3;
 
-static void plan_march2(Session &s) {
+void plan_march2(Session &s) {
     std::vector<MarchBlock> cls[3];
     march_classes2(s, s.tj2, s.seg2, cls);
     for (int c = 0; c < 3; c++) xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
@@ -1017,6 +1018,7 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
                 }
             if (s.seg2_try.size() < 2) s.seg2_try.clear();
         }
+        s.seg2_base = s.seg2;
         // the driven port's sources by plane (i), for the first step inside the launch
         std::vector<std::vector<uint4>> by_plane(P->n[0]);
         if (P->n_ports)
@@ -1323,20 +1325,16 @@ void set_step_counter(Session &s) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
 }
 
-// Time SEG2_TUNE_PAIRS two-step passes of each segment length in s.seg2_try (after one untimed
-// pass), keep the fastest and re-plan the march with it. The passes are steps of the run: every
-// tiling computes the same fields bit for bit. Returns the steps taken.
-static uint64_t tune_seg2(Session &s) {
+uint64_t seg2_tune_steps(const Session &s) {
+    return s.seg2_try.empty() ? 0 : 2 + 2ull * SEG2_TUNE_PAIRS * s.seg2_try.size();
+}
+
+uint64_t tune_seg2(Session &s, const std::function<void(uint64_t)> &chunk) {
     std::vector<uint32_t> cand;
     cand.swap(s.seg2_try);
     const uint32_t pairs = SEG2_TUNE_PAIRS;
     std::vector<hipEvent_t> ev(2 * cand.size());
     for (auto &e : ev) FG_CHECK(hipEventCreate(&e));
-    auto chunk = [&](uint64_t k) {
-        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
-        launch_steps(s, k);
-        s.steps += k;
-    };
     chunk(2);
     for (size_t c = 0; c < cand.size(); c++) {
         s.seg2 = cand[c];
@@ -1363,8 +1361,12 @@ static uint64_t tune_seg2(Session &s) {
 }
 
 void enqueue_steps(Session &s, uint64_t count) {
-    if (!s.timing && !s.seg2_try.empty() && count >= 2 + 2ull * SEG2_TUNE_PAIRS * s.seg2_try.size())
-        count -= tune_seg2(s);
+    if (!s.timing && !s.seg2_try.empty() && count >= seg2_tune_steps(s))
+        count -= tune_seg2(s, [&](uint64_t k) {
+            hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s.stream, s.step.p, uint32_t(s.steps));
+            launch_steps(s, k);
+            s.steps += k;
+        });
     if (s.timing) {
         collect_timing(s);
         // timed steps go in batches so the event pool stays bounded

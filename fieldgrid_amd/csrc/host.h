@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -226,6 +227,7 @@ struct Session {
     // segment lengths for the two-step march to time at the start of the run (enqueue_steps
     // picks the fastest and drops the list); empty: seg2 as set
     std::vector<uint32_t> seg2_try;
+    uint32_t seg2_base = 32;  // seg2 as set before any tuning
     DevBuf<MarchBlock> blk2_int, blk2_z, blk2_pml;
     DevBuf<MarchBlock> blk2_mix;  // the three, class-tagged, for k_march2_mix
     bool mix2 = true;             // one mixed launch per pass where the classes would run serially
@@ -338,6 +340,14 @@ void launch_step(Session &s, uint32_t s_off, uint64_t global_step);
 // join_classes makes the session stream wait for them.
 void fork_classes(Session &s, hipStream_t out[3]);
 void join_classes(Session &s);
+// The two-step march's block lists for segments of s.seg2 planes.
+void plan_march2(Session &s);
+// The segment tuner of the two-step march: times two-step passes of each length in s.seg2_try
+// (emptied) through chunk(k), which runs the session's next k steps (an even number), keeps the
+// fastest and re-plans the march with it. Every tiling computes the same fields bit for bit, so
+// the timed passes are steps of the run. Returns the steps taken, seg2_tune_steps(s) before.
+uint64_t tune_seg2(Session &s, const std::function<void(uint64_t)> &chunk);
+uint64_t seg2_tune_steps(const Session &s);
 // Two steps (s_off, s_off + 1) in one march2 launch; only when s.fuse2.
 void launch_pair(Session &s, uint32_t s_off);
 

@@ -301,6 +301,38 @@ def test_the_media_state_survives_the_snapshots(gpu):
 
 # ---- the whole tape: the forward tapes every step, the backward recomputes nothing ---------------
 
+def test_the_segment_tuner_leaves_the_gradient_alone(gpu, monkeypatch):
+    """A design run of 800 steps or more times the forward march's segment lengths on its first
+    passes (the checkpointed forward through enqueue_steps, the whole-tape forward on its taped
+    passes); the backward's recompute and adjoint sweep keep the untuned length. On the
+    192x192x112 design grid (16-plane segments untuned, so 16, 12, 8, 6 and 4 are timed) over
+    900 steps: series and gradient bit-identical, tuned or not (FIELDGRID_SEG2_TUNE=0),
+    checkpointed or with the whole tape, and a second backward equal to the first (the tuned
+    march restored after the recompute)."""
+    args, freq = W.design(steps=900)
+    dt = F.time_step(args["grid"])
+    out = {}
+    for tune in ("1", "0"):
+        monkeypatch.setenv("FIELDGRID_SEG2_TUNE", tune)
+        for whole in (False, True):
+            d = make(gpu, args, every=100, whole_tape=whole)
+            try:
+                s1 = d.forward()
+                _, dJ = W.dft_objective(s1[:, 0], dt, freq)
+                g1 = d.backward(dJ)
+                s2 = d.forward()
+                g2 = d.backward(dJ)
+                assert np.array_equal(s1, s2) and np.array_equal(g1, g2), (tune, whole)
+                out[(tune, whole)] = (s1, g1)
+            finally:
+                d.close()
+    s0, g0 = out[("0", False)]
+    assert np.max(np.abs(g0)) > 0
+    for key, (s1, g1) in out.items():
+        assert np.array_equal(s1, s0), key
+        assert np.array_equal(g1, g0), key
+
+
 def test_the_whole_tape_gives_the_checkpointed_gradient(gpu):
     """fg_design.whole_tape against checkpoint-and-recompute, bit for bit (series and gradient):
     the tape holds the same design-edge values whichever pass writes it, and the adjoint adds
