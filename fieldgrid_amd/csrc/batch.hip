@@ -117,15 +117,23 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_batch(const BatchRun 
     const BatchRun &r = runs[bb.run];
     const uint32_t n = r.fix.step[0] + s_off;
     const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
-    __shared__ March2Lds<TJ2, LDS_SETS, true> lds;
-    const MarchBlock mb{bb.bx, bb.by, bb.bz & 0x3FFFFFFFu};
+    __shared__ March2LdsEither<TJ2, LDS_SETS> lds;
+    const MarchBlock mb{bb.bx, bb.by, bb.bz & BZ_SEG};
     const int cls = PML >= 0 ? PML : int(bb.bz >> 30);
-#define FG_M2(C)                                                                                                    \
-    march2_ahead<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u], r.psi[par], \
-                                         r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{})
-    if (cls == 2) FG_M2(2);
-    else if (cls == 1) FG_M2(1);
-    else FG_M2(0);
+    // a block whose tile holds a Debye edge (march_classes2's SIDE_BLOCK) runs the in-pass ADE,
+    // every other block the lean march, which computes the same values there
+#define FG_M2(C, S, L)                                                                                                  \
+    march2_ahead<TJ2, C, WIDE, LDS_SETS, S>(L, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u], r.psi[par],  \
+                                            r.psi[par ^ 1u], seg, r.sr, pl, r.fix, r.db, Cap2{})
+    if (bb.bz & SIDE_BLOCK) {
+        if (cls == 2) FG_M2(2, true, lds.side);
+        else if (cls == 1) FG_M2(1, true, lds.side);
+        else FG_M2(0, true, lds.side);
+    } else {
+        if (cls == 2) FG_M2(2, false, lds.lean);
+        else if (cls == 1) FG_M2(1, false, lds.lean);
+        else FG_M2(0, false, lds.lean);
+    }
 #undef FG_M2
 }
 
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(64 * (TJ2 + 4)) void k_march2_persist(const BatchRu
             const uint32_t seg = __builtin_amdgcn_readfirstlane(A.seg);
             const uint32_t n = r.fix.step[0] + s_off;
             const float pl = n < r.fix.n_pulse ? r.fix.pulse[n] : pulse_at(r.fix, (float(n) + 0.5f) * r.fix.dt);
-            const MarchBlock mb{bb.bx, bb.by, bb.bz & 0x3FFFFFFFu};
+            const MarchBlock mb{bb.bx, bb.by, bb.bz & BZ_SEG};
             const int cls = int(bb.bz >> 30);
 #define FG_M2(C)                                                                                                    \
     march2_ahead<TJ2, C, WIDE, LDS_SETS, true>(lds, r.g, mb, r.E[par], r.E[par ^ 1u], r.H[par], r.H[par ^ 1u],    \
@@ -514,7 +522,7 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
         if (s.sheet.n)
             sl.push_back(r), b.sheet_gx = std::max(b.sheet_gx, s.sheet.serial ? 1u : uint32_t((s.sheet.n + 255) / 256));
         if (b.depth == 2) {
-            march_classes2(s, TJ2, b.seg2, cls);
+            march_classes2(s, TJ2, b.seg2, cls, 0, true);
             for (int c = 0; c < 3; c++)
                 for (const MarchBlock &m : cls[c]) L2[c * 4 + key].push_back(BatchBlock{m.bx, m.by, m.bz, r});
         }

@@ -919,6 +919,18 @@ struct March2Lds {
     uint4 xseg[CAP_SEGS];
 };
 
+// One LDS block for a kernel that runs either march of a block: with side work or lean.
+template <int TJ, bool LDS_SETS>
+union March2LdsEither {
+    March2Lds<TJ, LDS_SETS, true> side;
+    March2Lds<TJ, LDS_SETS, false> lean;
+};
+
+// Bit 29 of a two-step march block's bz (bits 30-31 carry the class in mixed lists): the block's
+// tile holds side work (march_classes2's side flags); kernels that run side work run the lean
+// march for a block without it, which computes the same values there.
+constexpr uint32_t SIDE_BLOCK = 1u << 29, BZ_SEG = SIDE_BLOCK - 1u;
+
 // SIDE: the first step has side work (Debye ADE through db, captures through cp; either may be
 // absent, NULL tables)
 template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool SIDE>

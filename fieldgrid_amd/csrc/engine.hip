@@ -25,6 +25,7 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -269,8 +270,20 @@ __global__ __launch_bounds__(256) void k_observe(Geo g, ObsArgs a, const float *
     observe(g, a, E, H, s_off, blockIdx.x, threadIdx.x);
 }
 
+// The LDS of a two-step march kernel: with side work (DEB) either march's, else the lean one's.
+template <int TJ, bool LDS_SETS, bool DEB>
+using March2Shared = std::conditional_t<DEB, March2LdsEither<TJ, LDS_SETS>, March2Lds<TJ, LDS_SETS, false>>;
+
+template <int TJ, bool LDS_SETS, bool DEB>
+__device__ __forceinline__ March2Lds<TJ, LDS_SETS, false> &lean_lds(March2Shared<TJ, LDS_SETS, DEB> &l) {
+    if constexpr (DEB) return l.lean;
+    else return l;
+}
+
 // Two steps of the march per launch (see march2): the first step's pulse value comes from the
-// device step counter and the host table, as in k_fix.
+// device step counter and the host table, as in k_fix. With side work (DEB), a block flagged
+// SIDE_BLOCK runs the side-work march and every other block the lean one (march2_ahead for the
+// interior class), which computes the same values where no side work falls.
 template <int TJ, int PML, bool WIDE, bool LDS_SETS, bool DEB>
 __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBlock *__restrict__ blocks,
                                                           const float *__restrict__ Eo, float *__restrict__ En,
@@ -278,13 +291,22 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2(Geo g, const MarchBloc
                                                           const float *__restrict__ psio, float *__restrict__ psin,
                                                           uint32_t seg, Src2 sr, FixArgs f, Deb2 db, Cap2 cp,
                                                           uint32_t s_off) {
-    const MarchBlock mb = blocks[blockIdx.x];
+    MarchBlock mb = blocks[blockIdx.x];
     if (mb.bx == ~0u) return;
+    const bool side = DEB && (mb.bz & SIDE_BLOCK);
+    mb.bz &= BZ_SEG;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
-    if constexpr (PML == 0 && !DEB) march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
-    else march2<TJ, PML, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    __shared__ March2Shared<TJ, LDS_SETS, DEB> lds;
+    if constexpr (DEB) {
+        if (side) {
+            march2<TJ, PML, WIDE, LDS_SETS, true>(lds.side, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+            return;
+        }
+    }
+    auto &ll = lean_lds<TJ, LDS_SETS, DEB>(lds);
+    if constexpr (PML == 0) march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(ll, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else march2<TJ, PML, WIDE, LDS_SETS, false>(ll, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
 // The same for a block list that mixes the CPML classes: the class is in the two top bits of
@@ -302,14 +324,23 @@ __global__ __launch_bounds__(64 * (TJ + 4)) void k_march2_mix(Geo g, const March
     MarchBlock mb = blocks[blockIdx.x];
     if (mb.bx == ~0u) return;
     const uint32_t cls = mb.bz >> 30;
-    mb.bz &= 0x3FFFFFFFu;
+    const bool side = DEB && (mb.bz & SIDE_BLOCK);
+    mb.bz &= BZ_SEG;
     const uint32_t n = f.step[0] + s_off;
     const float pl = n < f.n_pulse ? f.pulse[n] : pulse_at(f, (float(n) + 0.5f) * f.dt);
-    __shared__ March2Lds<TJ, LDS_SETS, DEB> lds;
-    if (cls == 2) march2<TJ, 2, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
-    else if (cls == 1) march2<TJ, 1, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
-    else if constexpr (!DEB) march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
-    else march2<TJ, 0, WIDE, LDS_SETS, DEB>(lds, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    __shared__ March2Shared<TJ, LDS_SETS, DEB> lds;
+    if constexpr (DEB) {
+        if (side) {
+            if (cls == 2) march2<TJ, 2, WIDE, LDS_SETS, true>(lds.side, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+            else if (cls == 1) march2<TJ, 1, WIDE, LDS_SETS, true>(lds.side, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+            else march2<TJ, 0, WIDE, LDS_SETS, true>(lds.side, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+            return;
+        }
+    }
+    auto &ll = lean_lds<TJ, LDS_SETS, DEB>(lds);
+    if (cls == 2) march2<TJ, 2, WIDE, LDS_SETS, false>(ll, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else if (cls == 1) march2<TJ, 1, WIDE, LDS_SETS, false>(ll, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
+    else march2_ahead<TJ, 0, WIDE, LDS_SETS, false>(ll, g, mb, Eo, En, Ho, Hn, psio, psin, seg, sr, pl, f, db, cp);
 }
 
 // Both steps' observers of a two-step pass (observe_pair): the first step exists only inside the
@@ -503,7 +534,20 @@ void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<Marc
             }
 }
 
-void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach) {
+// The 60-lane k tiles of the two-step march (mask bit t: tile t) that compute node k (halo: its
+// lanes k0-2 .. k0+61) or own it (k0 .. k0+59); with more than 64 tiles, all.
+static uint64_t side_tiles(uint32_t k, uint32_t n2, bool halo) {
+    if ((n2 + OWN_K2 - 1) / OWN_K2 > 64) return ~0ull;
+    uint64_t m = 1ull << (k / OWN_K2);
+    if (halo) {
+        if (k % OWN_K2 < 2 && k >= OWN_K2) m |= 1ull << (k / OWN_K2 - 1);  // the tile before reads it as k0+60, k0+61
+        if (k % OWN_K2 >= OWN_K2 - 2 && (k / OWN_K2 + 1) * OWN_K2 < n2) m |= 1ull << (k / OWN_K2 + 1);  // the next as k0-2, k0-1
+    }
+    return m;
+}
+
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach,
+                    bool side_flags) {
     const uint32_t nmax = s.nmax;
     std::vector<uint32_t> pre[3];
     for (int a = 0; a < 3; a++) {
@@ -520,6 +564,30 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
         return hi >= lo && pre[a][size_t(hi) + 1] > pre[a][size_t(lo)];
     };
     const uint32_t gx = (s.n[2] + OWN_K2 - 1) / OWN_K2, gy = (s.n[1] + tj - 1) / tj, gz = (s.n[0] + seg - 1) / seg;
+    // the blocks with side work: a Debye node in the planes (i0-2 .. i1), rows (j0-2 .. j0+tj+1)
+    // and lanes (k0-2 .. k0+61) march2 computes (its in-pass ADE runs on every copy of an edge),
+    // or a captured node it owns (i0 .. i1-1, j0 .. j0+tj-1, k0 .. k0+59): per (bz, by), the
+    // rows' k-tile masks ORed over those windows
+    std::vector<uint64_t> side;  // per (bz, by): the k tiles with side work
+    if (side_flags && (!s.side_deb.empty() || !s.side_cap.empty())) {
+        side.assign(size_t(gz) * gy, 0);
+        const int64_t n0 = s.n[0], n1 = s.n[1];
+        for (uint32_t bz = 0; bz < gz; bz++) {
+            const int64_t i0 = int64_t(bz) * seg, i1 = std::min<int64_t>(i0 + seg, n0);
+            for (uint32_t by = 0; by < gy; by++) {
+                const int64_t j0 = int64_t(by) * tj;
+                uint64_t m = 0;
+                if (!s.side_deb.empty())
+                    for (int64_t t = std::max<int64_t>(i0 - 2, 0); t <= std::min<int64_t>(i1, n0 - 1); t++)
+                        for (int64_t j = std::max<int64_t>(j0 - 2, 0); j < std::min<int64_t>(j0 + tj + 2, n1); j++)
+                            m |= s.side_deb[size_t(t * n1 + j)];
+                if (!s.side_cap.empty())
+                    for (int64_t t = i0; t < i1; t++)
+                        for (int64_t j = j0; j < std::min<int64_t>(j0 + tj, n1); j++) m |= s.side_cap[size_t(t * n1 + j)];
+                side[size_t(bz) * gy + by] = m;
+            }
+        }
+    }
     for (int c = 0; c < 3; c++) out[c].clear();
     for (uint32_t bz = 0; bz < gz; bz++)
         for (uint32_t by = 0; by < gy; by++)
@@ -528,7 +596,8 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
                 const int64_t i1 = std::min<int64_t>(i0 + seg, s.n[0]);
                 const bool z = touches(2, k0 - 2, k0 + OWN_K2 + 1);
                 const bool xy = touches(1, j0 - 2, j0 + tj + 1) || touches(0, i0 - 2, i1 + int64_t(reach));
-                out[xy ? 2 : z ? 1 : 0].push_back(MarchBlock{bx, by, bz});
+                const bool sw = !side.empty() && ((side[size_t(bz) * gy + by] >> std::min<uint32_t>(bx, 63)) & 1u);
+                out[xy ? 2 : z ? 1 : 0].push_back(MarchBlock{bx, by, bz | (sw ? SIDE_BLOCK : 0u)});
             }
 }
 
@@ -539,7 +608,7 @@ constexpr uint32_t SEG2_TUNE_PAIRS = 3;
 
 void plan_march2(Session &s) {
     std::vector<MarchBlock> cls[3];
-    march_classes2(s, s.tj2, s.seg2, cls);
+    march_classes2(s, s.tj2, s.seg2, cls, 0, true);
     for (int c = 0; c < 3; c++) xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
     s.blk2_int.upload(cls[0].data(), cls[0].size(), s.stream);
     s.blk2_z.upload(cls[1].data(), cls[1].size(), s.stream);
@@ -622,6 +691,8 @@ static void capture_points(Session &s, const fg_problem *P, const fg_extras *X, 
         if (!merged.empty() && merged.back().first == p.first) merged.back().second |= p.second;
         else merged.push_back(p);
     }
+    s.side_cap.assign(uint64_t(P->n[0]) * n1, 0);
+    for (const auto &p : merged) s.side_cap[p.first / n2] |= side_tiles(uint32_t(p.first % n2), uint32_t(n2), false);
     // segments: consecutive k of one row with one mask; node -> first index of its fields
     std::vector<uint32_t> start(uint64_t(P->n[0]) * n1 + 1, 0);
     std::vector<uint4> seg;
@@ -1056,6 +1127,11 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
             }
             s.deb2_row.upload(row.data(), row.size(), st);
             s.deb2_map.upload(map.data(), map.size(), st);
+            s.side_deb.assign(uint64_t(P->n[0]) * P->n[1], 0);
+            for (uint64_t t = 0; t < P->n_debye; t++) {
+                const uint64_t id = P->debye_edges[t].f % nn;
+                s.side_deb[id / P->n[2]] |= side_tiles(uint32_t(id % P->n[2]), P->n[2], true);
+            }
             s.deb2_jn.alloc(std::max<uint64_t>(uint64_t(P->n_poles) * P->n_debye, 1));
             s.deb2_eo.alloc(P->n_debye);
         }

@@ -240,6 +240,11 @@ struct Session {
     uint32_t cap_loop0 = 0, cap_probe0 = 0;
     DevBuf<uint4> cap_seg;
     DevBuf<float> cap_buf;
+    // the two-step march's side work per (i, j) row, as masks over its 60-lane k tiles (bit t:
+    // tile t): side_deb, the tiles whose lanes (k0-2 .. k0+61) hold a Debye node of the row;
+    // side_cap, the tiles that own a captured node of the row. Empty: no such work. With more
+    // than 64 k tiles every bit of a row with any such node is set. (march_classes2's flags)
+    std::vector<uint64_t> side_deb, side_cap;
     // resistive sheets (records restored from sheets0 on reset) and thin wires
     SheetArgs sheet{};
     DevBuf<float> sheets, sheet_cur, branches;
@@ -305,7 +310,11 @@ void march_classes(const Session &s, uint32_t tj, uint32_t seg, std::vector<Marc
 // The same for the two-step march (60 lanes x tj rows owned, two-cell halo, two-plane lag).
 // The two-step march's block classes; `reach` more planes past a segment's end are read (the
 // two-step adjoint reads one: its first step's lamH at the last plane needs lamE one plane on).
-void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach = 0);
+// side_flags: a block whose tile holds none of the session's side work (a Debye edge anywhere in
+// the planes, rows and lanes it computes, a captured node among those it owns) gets no flag; the
+// others get SIDE_BLOCK in bz, and the side-work kernels run the lean march for the former.
+void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<MarchBlock> out[3], uint32_t reach = 0,
+                    bool side_flags = false);
 // The largest per-plane byte count among the arrays the two-step march addresses plane by plane
 // (a field plane, one plane's rows of a z or y CPML psi slab): the argument of march2_seg_cap.
 inline uint64_t march2_plane_bytes(const uint32_t n[3], uint32_t pml) {
