@@ -300,6 +300,40 @@ def test_two_steps_per_launch_with_ntff_and_plane_dft_change_nothing(gpu, monkey
     _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"boxes with NTFF and plane DFT, seg2 {seg2}")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("seg2", ["16", "5"])
+def test_side_work_at_tile_edges_changes_nothing(gpu, monkeypatch, seg2):
+    """A side-work kernel runs the lean march for every block whose tile holds no Debye edge in
+    the planes, rows and lanes it computes and no captured point it owns. Debye slabs one and two
+    cells thick at the k-tile edges (cells 57-58 and 60-61 around the first tile's 60 owned lanes
+    and two-lane halo, 118-120 around the second's), across a row-tile edge (cells 11-12) and one
+    plane thick at a segment edge, with NTFF patches and a plane DFT: two steps per pass against
+    one step per pass and the FP32 oracle, bit for bit, at 16- and 5-plane segments."""
+    grid = W._grid([41, 50, 135], 1e-3, 6)
+    n = grid.dims()
+    cells = (n[0] - 1, n[1] - 1, n[2] - 1)
+    mats = F.Materials.new(grid)
+    eps, inf, dd = (getattr(mats, a).reshape(cells) for a in ("eps", "inf", "debye"))
+    for sl in ((slice(8, 30), slice(8, 40), slice(57, 59)), (slice(10, 13), slice(8, 40), slice(60, 62)),
+               (slice(15, 33), slice(11, 13), slice(100, 121)), (slice(16, 17), slice(20, 30), slice(20, 40))):
+        eps[sl], inf[sl], dd[sl] = 3.0, 3.0, 2.0
+    media = F.Media(debye=F.Debye([(1.0 / 20e-12, 1.0)]))
+    port = F.PortDef("p1", [[F.Edge(2, (20, 25, k)) for k in range(80, 84)]], 50.0, False)
+    sim = F.Sim.try_new(grid, mats, None, [], [port], [], media)
+    assert sim.problem().n_debye > 0
+    pulse = W.boxes_pulse()
+    ex = W.boxes_extras(sim, steps=351, inset=2, check_every=100, plane_k=70, probes=[(2, 30, 28, 90), (3, 25, 30, 31)])
+    monkeypatch.setenv("FIELDGRID_SEG2", seg2)
+    monkeypatch.delenv("FIELDGRID_FUSE2", raising=False)
+    two = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    monkeypatch.setenv("FIELDGRID_FUSE2", "0")
+    one = F.run(gpu, sim, 0, pulse, ex, fields=True)
+    for name in ("series", "probes", "plane", "ntff", "e", "h"):
+        a, b = getattr(two, name), getattr(one, name)
+        assert a.size > 0 and np.array_equal(a, b), f"{name}: {int(np.sum(a != b))} of {a.size} differ"
+    _exact(two, O.run_cpu(sim.desc, 0, pulse, ex, 32), f"side work at tile edges, seg2 {seg2}")
+
+
 # the two-step pass's dispatch: one launch over every block class (mixed), one launch per class
 # on the session stream (classes), or one per class on side streams forked from it (streams)
 _DISPATCH = {"mixed": {"FIELDGRID_MIX2": "1", "FIELDGRID_STREAMS": "0"},
