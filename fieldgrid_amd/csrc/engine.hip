@@ -769,9 +769,12 @@ Session *session_new(fg_device *dev, const fg_problem *P, uint32_t driven, const
     Stager stage(st);               // the tables below go up through pinned memory
     ArenaScope arena_scope(s.arena);  // and the small ones into the session's arena
     // concurrent block classes: on a 16.7 M-node grid the classes' last rounds overlap for ~10 %;
-    // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves
+    // below ~8 M nodes a pass lasts ~100 us and the fork/join latency costs more than it saves,
+    // and from ~16 blocks per CU (32-plane two-step blocks) the one mixed launch, longest blocks
+    // first, wins (768x768x384, 42 blocks per CU: 2.247 against 2.318 ms per step)
     const char *cc = getenv("FIELDGRID_STREAMS");
-    const bool side = cc ? cc[0] != '0' : nn >= (8ull << 20);
+    const uint64_t blocks32 = uint64_t((P->n[2] + OWN_K2 - 1) / OWN_K2) * ((P->n[1] + 11) / 12) * ((P->n[0] + 31) / 32);
+    const bool side = cc ? cc[0] != '0' : nn >= (8ull << 20) && blocks32 < 16ull * uint64_t(std::max(dev->cus, 1));
     if (const char *m = getenv("FIELDGRID_MIX2")) s.mix2 = m[0] != '0';
     if (s.fused && side) {
         for (int q = 0; q < 2; q++) {
