@@ -370,7 +370,8 @@ def single(args):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "alg_bytes_per_launch": round(alg), "steps_per_launch": per,
-            "kernel": ("k_march2: two steps per launch (interior, z-CPML, CPML classes)"
+            "kernel": ("k_march2_mix or k_march2: two steps per launch over the interior, z-CPML and x/y-CPML "
+                       "blocks (one mixed launch from 16 blocks per CU or below 8 M nodes, else one per class)"
                        if two else "k_march sweep (interior, z-CPML, CPML launches)" if args.engine == "fused"
                        else "k_update_h + k_update_e"),
             "kernel_ms": round(launch_ms, 4), "alg_bytes_per_node_step": round(core_bpn, 3),
