@@ -172,6 +172,13 @@ def sweep_leg(args):
         kernel = "k_march2_batch<-1> (class-mixed lists, two steps per pass) + k_fix_batch + k_observe_pair_batch"
     else:
         kernel = "k_march_batch (one launch per class, one step per pass) + k_fix_batch + k_observe_batch"
+    # measured HBM bytes per step of a full-width two-step pass, from the committed rocprofv3 PMC
+    # summary of this sweep (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_march_sweep_traffic.json")
+    if depth == 2 and tune.used_launch != 2 and os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("traffic_bytes_per_step")
     obj = {"runs": len(runs), "node_updates": node_steps, "batched_s": round(bat_s, 3), "sequential_s": round(seq_s, 3),
            "speedup": round(seq_s / bat_s, 3), "batched_mcells": round(node_steps / bat_s / 1e6, 2),
            "sequential_mcells": round(node_steps / seq_s / 1e6, 2), "records_identical": same,
@@ -181,10 +188,15 @@ def sweep_leg(args):
                     "tune_ms": round(tune.tune_ms, 1), "kept_steps": tune.kept_steps, "from_cache": tune.from_cache},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                        "traffic": None, "alg_bytes_per_step": round(alg), "kernel": kernel,
+                        "traffic": traffic, "alg_bytes_per_step": round(alg), "kernel": kernel,
                         "step_ms": round(tune.step_ms, 4),
                         "note": "effective bandwidth: one-step algorithmic bytes of all 96 runs over the "
                                 "tuner's HIP-event time per step of the whole batch"}}
+    if traffic and tune.step_ms > 0:
+        roof = obj["roofline"]
+        roof["traffic_gbs"] = round(traffic / (tune.step_ms * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_gbs"] / HBM_PEAK_GBS, 4)
+        roof["traffic_over_alg"] = round(traffic / alg, 3)
     return obj, node_steps / bat_s / 1e6, same
 
 
