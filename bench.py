@@ -266,6 +266,12 @@ def adjoint_leg(args, fd_cells):
     alg = core * nn + 44.0 * edges
     adj_step_ms = adj_ms / a["steps"]
     achieved = alg / (adj_step_ms * 1e-3) / 1e9
+    # measured HBM bytes per adjoint step, from the committed rocprofv3 PMC summary of this problem
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_adj_design_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("traffic_bytes_per_step")
     obj = {"dims": list(n), "steps": a["steps"], "design_box_cells": box, "objective": "|DFT Ez at 8 GHz|^2",
            "snapshot_every": 100, "snapshots": snaps, "forward_ms": round(fwd_ms, 1), "backward_ms": round(bwd_ms, 1),
            "backward_over_forward": round(bwd_ms / fwd_ms, 3), "recompute_ms": round(rec_ms, 1),
@@ -280,10 +286,15 @@ def adjoint_leg(args, fd_cells):
                           "note": "the forward tapes the design edges of every step; the backward runs the adjoint "
                                   "alone, with no snapshots or recompute"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": round(alg),
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "alg_bytes_per_step": round(alg),
                         "kernel": ("k_adj_march2 (two adjoint steps per launch, per CPML class; HIP-event busy time of "
                                    "its stream per step, the recompute overlapped on a second stream)"),
                         "step_ms": round(adj_step_ms, 4)}}
+    if traffic:
+        roof = obj["roofline"]
+        roof["traffic_gbs"] = round(traffic / (adj_step_ms * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_gbs"] / HBM_PEAK_GBS, 4)
+        roof["traffic_over_alg"] = round(traffic / alg, 3)
     return obj, nn * a["steps"] / (fwd_ms * 1e-3) / 1e6
 
 
