@@ -283,8 +283,11 @@ def adjoint_leg(args, fd_cells):
                           "tape_gb": round(8.0 * edges * a["steps"] / 1e9, 2),
                           "series_identical": bool(np.array_equal(w_series, series)),
                           "grad_identical": bool(np.array_equal(w_grad, grad)),
+                          "adjoint_step_gbs": round(alg / (w_bwd / a["steps"] * 1e-3) / 1e9, 1),
+                          "adjoint_step_frac": round(alg / (w_bwd / a["steps"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "note": "the forward tapes the design edges of every step; the backward runs the adjoint "
-                                  "alone, with no snapshots or recompute"},
+                                  "alone, with no snapshots or recompute (adjoint_step_*: algorithmic bytes of an "
+                                  "adjoint step over the backward's time per step)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "alg_bytes_per_step": round(alg),
                         "kernel": ("k_adj_march2 (two adjoint steps per launch, per CPML class; HIP-event busy time of "
