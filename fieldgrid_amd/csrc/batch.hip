@@ -532,6 +532,7 @@ static void plan(Batch &b, const std::vector<uint8_t> &active) {
     if (b.depth == 2 && b.persist) plan_persist(b, active);
     for (int q = 0; q < N_LISTS; q++) {
         xcd_order(L[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
+        first_per_xcd(L2[q], b.xcds, [](const BatchBlock &m) { return (m.bz & SIDE_BLOCK) != 0; });
         xcd_order(L2[q], b.xcds, BatchBlock{~0u, ~0u, ~0u, ~0u});
     }
     // the two-step march runs each key's three classes in one mixed list (k_march2_batch<-1>),

@@ -609,7 +609,10 @@ constexpr uint32_t SEG2_TUNE_PAIRS = 3;
 void plan_march2(Session &s) {
     std::vector<MarchBlock> cls[3];
     march_classes2(s, s.tj2, s.seg2, cls, 0, true);
-    for (int c = 0; c < 3; c++) xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
+    for (int c = 0; c < 3; c++) {
+        first_per_xcd(cls[c], s.xcds, [](const MarchBlock &m) { return (m.bz & SIDE_BLOCK) != 0; });
+        xcd_order(cls[c], s.xcds, MarchBlock{~0u, ~0u, ~0u});
+    }
     s.blk2_int.upload(cls[0].data(), cls[0].size(), s.stream);
     s.blk2_z.upload(cls[1].data(), cls[1].size(), s.stream);
     s.blk2_pml.upload(cls[2].data(), cls[2].size(), s.stream);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <mutex>
@@ -320,6 +321,16 @@ void march_classes2(const Session &s, uint32_t tj, uint32_t seg, std::vector<Mar
 inline uint64_t march2_plane_bytes(const uint32_t n[3], uint32_t pml) {
     const uint64_t p2 = 2ull * pml, a = uint64_t(n[1]) * n[2], b = uint64_t(n[1]) * p2, c = p2 * n[2];
     return 4ull * (a > b ? (a > c ? a : c) : (b > c ? b : c));
+}
+
+// Within the contiguous run of a block list that xcd_order hands each XCD, the blocks for which
+// `first` holds go first (stable otherwise): the side-work blocks of the two-step march, which
+// take longer, start in the first round instead of trailing the pass.
+template <class T, class F>
+void first_per_xcd(std::vector<T> &v, size_t xcds, F first) {
+    const size_t N = v.size(), X = xcds > 1 ? xcds : 1, per = (N + X - 1) / X;
+    for (size_t x = 0; x * per < N; x++)
+        std::stable_partition(v.begin() + x * per, v.begin() + std::min(N, (x + 1) * per), first);
 }
 
 // Workgroups go to the XCDs round-robin (launch index b runs on XCD b % xcds). Permute a block
